@@ -1,0 +1,409 @@
+// C-ABI of libeuler2d_hip: contexts, validation, status codes, and the
+// host-side flow of the reference's HydroRun module (cuf/m_HydroRun_gpu.cuf,
+// serial_ref/m_run.f90) over the gfx950 kernels.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "../../include/euler2d_hip.h"
+#include "e2d_internal.hpp"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                            \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess) return fail(E2D_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// SplitMix64 (Steele, Lea, Flood 2014): seeds the Gaussian bumps.
+uint64_t splitmix64(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+double uniform01(uint64_t& s) { return static_cast<double>(splitmix64(s) >> 11) * 0x1.0p-53; }
+
+int validate(const e2d_params* p) {
+  if (!p) return fail(E2D_ERR_ARG, "params is NULL");
+  if (p->nx < 4 || p->ny < 4) return fail(E2D_ERR_ARG, "nx and ny must be >= 4 (got %d, %d)", p->nx, p->ny);
+  if (static_cast<int64_t>(p->nx + 4) * (p->ny + 4) > (int64_t(1) << 31) / 4)
+    return fail(E2D_ERR_ARG, "grid %d x %d too large", p->nx, p->ny);
+  if (p->implementation_version != E2D_IMPL_FUSED && p->implementation_version != E2D_IMPL_MATERIALISED)
+    return fail(E2D_ERR_ARG,
+                "The implementation version parameter should be 0 or 1 (got %d); check the namelist, section OTHER",
+                p->implementation_version);
+  if (p->precision < E2D_PREC_FP64 || p->precision > E2D_PREC_MIXED)
+    return fail(E2D_ERR_ARG, "precision must be 0 (fp64), 1 (fp32) or 2 (mixed), got %d", p->precision);
+  if (p->riemann_solver != E2D_RIEMANN_APPROX && p->riemann_solver != E2D_RIEMANN_HLLC)
+    return fail(E2D_ERR_ARG, "riemann_solver must be 0 (approx) or 2 (hllc), got %d", p->riemann_solver);
+  if (p->niter_riemann < 0) return fail(E2D_ERR_ARG, "niter_riemann must be >= 0");
+  if (!(p->gamma0 > 1.0)) return fail(E2D_ERR_ARG, "gamma0 must be > 1");
+  if (!(p->cfl > 0.0)) return fail(E2D_ERR_ARG, "cfl must be > 0");
+  if (!(p->smallr > 0.0) || !(p->smallc > 0.0)) return fail(E2D_ERR_ARG, "smallr and smallc must be > 0");
+  return E2D_OK;
+}
+
+}  // namespace
+
+struct e2d_ctx {
+  e2d_params p{};
+  e2d_ctx() = default;
+  e2d::DevBuffers b;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int isize = 0, jsize = 0;
+  double* staging = nullptr;  // float64 host-layout staging array on the device
+  double* d_sums = nullptr;
+  // running invDt of compute_dt: authoritative on the host when dev_slot < 0,
+  // else in b.scal[dev_slot]
+  double invdt = 0.0;
+  int dev_slot = -1;
+  bool stages_valid = false;
+};
+
+namespace {
+
+int check_ctx(const e2d_ctx* c) {
+  if (!c) return fail(E2D_ERR_ARG, "context is NULL");
+  return E2D_OK;
+}
+int check_buf(int buf) {
+  if (buf != 0 && buf != 1) return fail(E2D_ERR_ARG, "buffer index must be 0 (u) or 1 (u2), got %d", buf);
+  return E2D_OK;
+}
+
+size_t host_elems(const e2d_ctx* c) { return static_cast<size_t>(c->isize) * c->jsize * 4; }
+
+// bring the running invDt to the host
+int pull_invdt(e2d_ctx* c) {
+  if (c->dev_slot >= 0) {
+    HIP_TRY(hipMemcpyAsync(&c->invdt, c->b.scal + c->dev_slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->dev_slot = -1;
+  }
+  return E2D_OK;
+}
+
+// dt = cfl / invDt in the run's precision (serial_ref/m_run.f90:308)
+double dt_of(const e2d_params& p, double invdt) {
+  if (p.precision == E2D_PREC_FP32) return static_cast<double>(static_cast<float>(p.cfl) / static_cast<float>(invdt));
+  return p.cfl / invdt;
+}
+
+void free_all(e2d_ctx* c) {
+  for (auto& u : c->b.u)
+    if (u) (void)hipFree(u), u = nullptr;
+  for (auto& s : c->b.stage)
+    if (s) (void)hipFree(s), s = nullptr;
+  if (c->b.scal) (void)hipFree(c->b.scal), c->b.scal = nullptr;
+  if (c->b.partial) (void)hipFree(c->b.partial), c->b.partial = nullptr;
+  if (c->staging) (void)hipFree(c->staging), c->staging = nullptr;
+  if (c->d_sums) (void)hipFree(c->d_sums), c->d_sums = nullptr;
+  if (c->stream) (void)hipStreamDestroy(c->stream), c->stream = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int e2d_version(void) { return E2D_VERSION; }
+
+int e2d_device_available(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n > 0 ? 1 : 0;
+}
+
+const char* e2d_last_error(void) { return g_err; }
+
+int e2d_create(const e2d_params* p, e2d_ctx** out) {
+  g_err[0] = '\0';
+  if (!out) return fail(E2D_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  int rc = validate(p);
+  if (rc) return rc;
+  if (!e2d_device_available()) return fail(E2D_ERR_HIP, "no HIP device available");
+  e2d_ctx* c = new (std::nothrow) e2d_ctx();
+  if (!c) return fail(E2D_ERR_NOMEM, "host allocation failed");
+  c->p = *p;
+  c->isize = p->nx + 4;
+  c->jsize = p->ny + 4;
+  HIP_TRY(hipGetDevice(&c->device));
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(E2D_ERR_HIP, "hipStreamCreate failed");
+  }
+  // rows padded to a multiple of 64 elements (256 B for FP32, 512 B for FP64)
+  c->b.pitch = (static_cast<int64_t>(c->isize) + 63) / 64 * 64;
+  c->b.plane = c->b.pitch * c->jsize;
+  const size_t n = static_cast<size_t>(c->b.plane) * 4;
+  const size_t es = e2d::state_elem_bytes(p->precision);
+  bool ok = hipMalloc(&c->b.u[0], n * es) == hipSuccess && hipMalloc(&c->b.u[1], n * es) == hipSuccess &&
+            hipMalloc(reinterpret_cast<void**>(&c->b.scal), 4 * sizeof(double)) == hipSuccess &&
+            hipMalloc(reinterpret_cast<void**>(&c->b.partial), e2d::kSumBlocks * 4 * sizeof(double)) == hipSuccess &&
+            hipMalloc(reinterpret_cast<void**>(&c->d_sums), 4 * sizeof(double)) == hipSuccess;
+  if (ok && p->implementation_version == E2D_IMPL_MATERIALISED) {
+    for (int s = 0; s < e2d::kNumStages && ok; ++s) {
+      const size_t bytes = n * e2d::stage_elem_bytes(p->precision, s);
+      ok = hipMalloc(&c->b.stage[s], bytes) == hipSuccess &&
+           hipMemsetAsync(c->b.stage[s], 0, bytes, c->stream) == hipSuccess;
+    }
+  }
+  if (!ok) {
+    free_all(c);
+    delete c;
+    return fail(E2D_ERR_NOMEM, "device allocation failed (%zu bytes per state buffer)", n * es);
+  }
+  // both buffers start at zero (u2 as the reference leaves it), padding included
+  if (hipMemsetAsync(c->b.u[0], 0, n * es, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->b.u[1], 0, n * es, c->stream) != hipSuccess ||
+      e2d::launch_fill(c->b.scal, 4, 0.0, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
+    free_all(c);
+    delete c;
+    return fail(E2D_ERR_HIP, "device initialisation failed");
+  }
+  *out = c;
+  return E2D_OK;
+}
+
+int e2d_destroy(e2d_ctx* c) {
+  if (!c) return E2D_OK;
+  (void)hipStreamSynchronize(c->stream);
+  free_all(c);
+  delete c;
+  return E2D_OK;
+}
+
+int e2d_init_problem(e2d_ctx* c, int problem, uint64_t seed) {
+  if (int rc = check_ctx(c)) return rc;
+  if (problem < E2D_PROBLEM_IMPLODE || problem > E2D_PROBLEM_BUMPS)
+    return fail(E2D_ERR_ARG, "unknown problem %d", problem);
+  double bumps[16 * 5];
+  uint64_t s = seed;
+  for (int k = 0; k < 16; ++k) {
+    bumps[5 * k + 0] = uniform01(s);               // centre x
+    bumps[5 * k + 1] = uniform01(s);               // centre y
+    bumps[5 * k + 2] = 0.1 + 1.9 * uniform01(s);   // amplitude on rho, U[0.1, 2]
+    bumps[5 * k + 3] = 0.1 + 1.9 * uniform01(s);   // amplitude on p,   U[0.1, 2]
+    bumps[5 * k + 4] = 0.02 + 0.08 * uniform01(s); // sigma,            U[0.02, 0.1]
+  }
+  HIP_TRY(e2d::launch_init_problem(c->p, c->b, problem, bumps, c->stream));
+  c->invdt = 0.0;
+  c->dev_slot = -1;
+  c->stages_valid = false;
+  return E2D_OK;
+}
+
+int e2d_upload(e2d_ctx* c, int buf, const double* h) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_buf(buf)) return rc;
+  if (!h) return fail(E2D_ERR_ARG, "host array is NULL");
+  if (!c->staging)
+    HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->staging), host_elems(c) * sizeof(double)));
+  HIP_TRY(hipMemcpyAsync(c->staging, h, host_elems(c) * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(e2d::launch_pack(c->p, c->b, buf, c->staging, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return E2D_OK;
+}
+
+static int download_array(e2d_ctx* c, const void* d_state, int elem_bytes, double* h) {
+  if (!c->staging)
+    HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->staging), host_elems(c) * sizeof(double)));
+  HIP_TRY(e2d::launch_unpack(c->p, c->b, d_state, elem_bytes, c->staging, c->stream));
+  HIP_TRY(hipMemcpyAsync(h, c->staging, host_elems(c) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return E2D_OK;
+}
+
+int e2d_download(e2d_ctx* c, int buf, double* h) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_buf(buf)) return rc;
+  if (!h) return fail(E2D_ERR_ARG, "host array is NULL");
+  return download_array(c, c->b.u[buf], static_cast<int>(e2d::state_elem_bytes(c->p.precision)), h);
+}
+
+int e2d_download_stage(e2d_ctx* c, int stage, double* h) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!h) return fail(E2D_ERR_ARG, "host array is NULL");
+  if (stage < 0 || stage >= e2d::kNumStages) return fail(E2D_ERR_ARG, "unknown stage %d", stage);
+  if (c->p.implementation_version != E2D_IMPL_MATERIALISED)
+    return fail(E2D_ERR_STATE, "stage arrays exist only with implementationVersion 1");
+  return download_array(c, c->b.stage[stage], static_cast<int>(e2d::stage_elem_bytes(c->p.precision, stage)), h);
+}
+
+int e2d_compute_dt(e2d_ctx* c, int useU, double* dt) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!dt) return fail(E2D_ERR_ARG, "dt is NULL");
+  if (int rc = pull_invdt(c)) return rc;
+  const int buf = (useU == 0) ? 0 : 1;
+  HIP_TRY(e2d::launch_fill(c->b.scal + 2, 1, c->invdt, c->stream));
+  HIP_TRY(e2d::launch_max_invdt(c->p, c->b, buf, 2, c->stream));
+  HIP_TRY(hipMemcpyAsync(&c->invdt, c->b.scal + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  *dt = dt_of(c->p, c->invdt);
+  return E2D_OK;
+}
+
+int e2d_make_boundaries(e2d_ctx* c, int buf) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_buf(buf)) return rc;
+  HIP_TRY(e2d::launch_boundaries(c->p, c->b, buf, c->stream));
+  return E2D_OK;
+}
+
+int e2d_godunov_unsplit(e2d_ctx* c, int nStep, double dt) {
+  if (int rc = check_ctx(c)) return rc;
+  if (nStep < 0) return fail(E2D_ERR_ARG, "nStep must be >= 0");
+  if (int rc = pull_invdt(c)) return rc;
+  e2d::StepCfg cfg;
+  cfg.buf_in = nStep % 2;
+  cfg.buf_out = 1 - cfg.buf_in;
+  cfg.dt_from_device = false;
+  cfg.dt_host = dt;
+  cfg.track_dt = false;
+  if (c->p.implementation_version == E2D_IMPL_FUSED) {
+    HIP_TRY(e2d::launch_fused_step(c->p, c->b, cfg, c->stream));
+  } else {
+    HIP_TRY(e2d::launch_materialised_step(c->p, c->b, cfg, c->stream));
+    c->stages_valid = true;
+  }
+  return E2D_OK;
+}
+
+int e2d_run_steps(e2d_ctx* c, int nsteps, int* nStep) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!nStep) return fail(E2D_ERR_ARG, "nStep is NULL");
+  if (nsteps < 0 || *nStep < 0) return fail(E2D_ERR_ARG, "nsteps and *nStep must be >= 0");
+  if (nsteps == 0) return E2D_OK;
+  int n = *nStep;
+  int k = c->dev_slot;
+  if (k < 0) {
+    // compute_dt of the current input, folded into the running maximum
+    k = 0;
+    HIP_TRY(e2d::launch_fill(c->b.scal + k, 1, c->invdt, c->stream));
+    HIP_TRY(e2d::launch_max_invdt(c->p, c->b, n % 2, k, c->stream));
+  }
+  e2d::StepCfg cfg;
+  cfg.dt_from_device = true;
+  if (c->p.implementation_version == E2D_IMPL_FUSED) {
+    // slot k holds the running invDt including the current input; the other
+    // slot must not exceed it before the first step writes there
+    HIP_TRY(e2d::launch_fill(c->b.scal + (1 - k), 1, 0.0, c->stream));
+    for (int s = 0; s < nsteps; ++s, ++n) {
+      cfg.buf_in = n % 2;
+      cfg.buf_out = 1 - cfg.buf_in;
+      cfg.slot_cur = k;
+      cfg.slot_next = 1 - k;
+      cfg.track_dt = true;
+      HIP_TRY(e2d::launch_fused_step(c->p, c->b, cfg, c->stream));
+      k = 1 - k;
+    }
+  } else {
+    for (int s = 0; s < nsteps; ++s, ++n) {
+      cfg.buf_in = n % 2;
+      cfg.buf_out = 1 - cfg.buf_in;
+      cfg.slot_cur = k;
+      HIP_TRY(e2d::launch_materialised_step(c->p, c->b, cfg, c->stream));
+      HIP_TRY(e2d::launch_max_invdt(c->p, c->b, cfg.buf_out, k, c->stream));
+    }
+    c->stages_valid = true;
+  }
+  c->dev_slot = k;
+  *nStep = n;
+  return E2D_OK;
+}
+
+int e2d_get_dt(e2d_ctx* c, double* dt) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!dt) return fail(E2D_ERR_ARG, "dt is NULL");
+  if (int rc = pull_invdt(c)) return rc;
+  *dt = dt_of(c->p, c->invdt);
+  return E2D_OK;
+}
+
+int e2d_get_invdt(e2d_ctx* c, double* invdt) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!invdt) return fail(E2D_ERR_ARG, "invdt is NULL");
+  if (int rc = pull_invdt(c)) return rc;
+  *invdt = c->invdt;
+  return E2D_OK;
+}
+
+int e2d_set_invdt(e2d_ctx* c, double invdt) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!(invdt >= 0.0)) return fail(E2D_ERR_ARG, "invdt must be >= 0");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->invdt = invdt;
+  c->dev_slot = -1;
+  return E2D_OK;
+}
+
+int e2d_sums(e2d_ctx* c, int buf, double out[4]) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_buf(buf)) return rc;
+  if (!out) return fail(E2D_ERR_ARG, "out is NULL");
+  HIP_TRY(e2d::launch_sums(c->p, c->b, buf, c->d_sums, c->stream));
+  HIP_TRY(hipMemcpyAsync(out, c->d_sums, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return E2D_OK;
+}
+
+int e2d_synchronize(e2d_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return E2D_OK;
+}
+
+int e2d_device_buffer(e2d_ctx* c, int buf, void** dptr, int64_t* pitch) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_buf(buf)) return rc;
+  if (!dptr || !pitch) return fail(E2D_ERR_ARG, "output pointer is NULL");
+  *dptr = c->b.u[buf];
+  *pitch = c->b.pitch;
+  return E2D_OK;
+}
+
+int e2d_pointwise(const e2d_params* p, int func, int n, const double* x, double* y) {
+  g_err[0] = '\0';
+  if (!p) return fail(E2D_ERR_ARG, "params is NULL");
+  if (func < E2D_FN_PRIM || func > E2D_FN_EOS) return fail(E2D_ERR_ARG, "unknown function %d", func);
+  if (n < 0 || (n > 0 && (!x || !y))) return fail(E2D_ERR_ARG, "bad batch");
+  if (p->precision < E2D_PREC_FP64 || p->precision > E2D_PREC_MIXED) return fail(E2D_ERR_ARG, "bad precision");
+  if (n == 0) return E2D_OK;
+  if (!e2d_device_available()) return fail(E2D_ERR_HIP, "no HIP device available");
+  static const int nin[8] = {0, 4, 20, 15, 8, 8, 4, 2};
+  static const int nout[8] = {0, 5, 8, 4, 8, 8, 4, 2};
+  double *dx = nullptr, *dy = nullptr;
+  HIP_TRY(hipMalloc(reinterpret_cast<void**>(&dx), sizeof(double) * n * nin[func]));
+  if (hipMalloc(reinterpret_cast<void**>(&dy), sizeof(double) * n * nout[func]) != hipSuccess) {
+    (void)hipFree(dx);
+    return fail(E2D_ERR_NOMEM, "device allocation failed");
+  }
+  int rc = E2D_OK;
+  if (hipMemcpy(dx, x, sizeof(double) * n * nin[func], hipMemcpyHostToDevice) != hipSuccess ||
+      e2d::launch_pointwise(*p, func, n, dx, dy, nullptr) != hipSuccess ||
+      hipMemcpy(y, dy, sizeof(double) * n * nout[func], hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(E2D_ERR_HIP, "pointwise batch failed: %s", hipGetErrorString(hipGetLastError()));
+  (void)hipFree(dx);
+  (void)hipFree(dy);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,61 @@
+// Host <-> kernel interface inside libeuler2d_hip (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/euler2d_hip.h"
+
+namespace e2d {
+
+constexpr int kNumStages = 9;  // E2D_STAGE_Q .. E2D_STAGE_QYP
+
+// Device memory of one run.  State element (i, j, v), 0-based, of buffer b
+// sits at u[b][v * plane + j * pitch + i].
+struct DevBuffers {
+  void* u[2] = {nullptr, nullptr};
+  int64_t pitch = 0;  // elements per row (>= isize, padded)
+  int64_t plane = 0;  // elements per variable plane (pitch * jsize)
+  void* stage[kNumStages] = {};
+  // device scalars: [0], [1] running-invDt slots (ping-pong), [2] reduction
+  // target of a standalone compute_dt
+  double* scal = nullptr;
+  double* partial = nullptr;  // per-block partial sums (kSumBlocks * 4)
+};
+
+constexpr int kSumBlocks = 240;
+
+// How a step obtains dt.
+struct StepCfg {
+  int buf_in = 0, buf_out = 1;
+  bool dt_from_device = false;  // dt = cfl / scal[slot_cur]; else dt_host
+  double dt_host = 0.0;
+  int slot_cur = 0;
+  bool track_dt = false;  // fold the new state's CFL rate into scal[slot_next]
+  int slot_next = 1;
+};
+
+size_t state_elem_bytes(int precision);   // storage type of the state
+size_t stage_elem_bytes(int precision, int stage);
+
+hipError_t launch_fused_step(const e2d_params& p, const DevBuffers& b, const StepCfg& c, hipStream_t s);
+hipError_t launch_materialised_step(const e2d_params& p, const DevBuffers& b, const StepCfg& c,
+                                    hipStream_t s);
+hipError_t launch_boundaries(const e2d_params& p, const DevBuffers& b, int buf, hipStream_t s);
+// atomic max of the CFL rate over the compute_dt window of buffer buf into scal[slot]
+hipError_t launch_max_invdt(const e2d_params& p, const DevBuffers& b, int buf, int slot, hipStream_t s);
+hipError_t launch_fill(double* dst, int n, double v, hipStream_t s);
+hipError_t launch_init_problem(const e2d_params& p, const DevBuffers& b, int problem,
+                               const double* bumps /* 16 x 5 host values */, hipStream_t s);
+hipError_t launch_zero_state(const e2d_params& p, const DevBuffers& b, int buf, hipStream_t s);
+hipError_t launch_sums(const e2d_params& p, const DevBuffers& b, int buf, double* d_out4, hipStream_t s);
+// conversion between the float64 host layout (isize*jsize*4, Fortran order)
+// and the pitched device state, through a device staging array of doubles
+hipError_t launch_pack(const e2d_params& p, const DevBuffers& b, int buf, const double* d_src, hipStream_t s);
+hipError_t launch_unpack(const e2d_params& p, const DevBuffers& b, const void* d_state, int elem_kind,
+                         double* d_dst, hipStream_t s);
+hipError_t launch_pointwise(const e2d_params& p, int func, int n, const double* d_x, double* d_y,
+                            hipStream_t s);
+
+}  // namespace e2d

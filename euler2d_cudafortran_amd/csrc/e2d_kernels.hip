@@ -1,0 +1,965 @@
+// gfx950 kernels of the euler2d hot path.
+//
+// Fused step (implementationVersion 0): one kernel per time step.  Each
+// wavefront owns a strip of 64 consecutive columns (60 output columns and a
+// 2-column halo each side) and marches upward through a segment of rows.  The
+// y-neighbourhood of a cell stays in the lane's registers (rolling rows of
+// primitive variables); x-neighbours come from adjacent lanes through
+// cross-lane shuffles, so no LDS tile is needed.  Per cell the primitive
+// variables, slopes and trace source terms are computed once; per face one
+// Riemann problem is solved.  Ghost cells are produced on load from their
+// interior source (make_boundaries, serial_ref/m_run.f90:561-648) and written
+// back to the input buffer, so buffers end up exactly as the reference leaves
+// them.  The update epilogue also folds the CFL rate of the new state into the
+// running invDt of the next step (compute_dt, serial_ref/m_run.f90:248-312).
+//
+// Materialised step (implementationVersion 1): boundaries, primitive
+// variables, slopes, face states, fluxes and update as separate kernels with
+// every stage stored in HBM (cuf/m_HydroRun_gpu.cuf:393-418 structure, with
+// serial_ref arithmetic).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "e2d_internal.hpp"
+#include "e2d_numerics.hpp"
+
+namespace e2d {
+
+// ---------------------------------------------------------------------------
+// precision policies: storage S, flux arithmetic T, update / dt type U, math M
+template <class S_, class T_, class U_, class M_>
+struct Prec {
+  using S = S_;
+  using T = T_;
+  using U = U_;
+  using M = M_;
+};
+using P64 = Prec<double, double, double, ExactMath>;
+using P32 = Prec<float, float, float, ExactMath>;
+using PMix = Prec<float, float, double, FastMath>;
+
+// Derived parameters in precision T, as initHydroParameters computes them
+// (serial_ref/m_run.f90:111-124) plus the cmpflx constant entho (m_utils.f90:521).
+template <class T>
+static Hydro<T> make_hydro(const e2d_params& p) {
+  Hydro<T> h;
+  h.gamma0 = static_cast<T>(p.gamma0);
+  h.cfl = static_cast<T>(p.cfl);
+  h.smallr = static_cast<T>(p.smallr);
+  h.smallc = static_cast<T>(p.smallc);
+  h.smallp = h.smallc * h.smallc / h.gamma0;
+  h.smallpp = h.smallr * h.smallp;
+  h.gamma6 = (h.gamma0 + T(1)) / (T(2) * h.gamma0);
+  h.entho = T(1) / (h.gamma0 - T(1));
+  h.dx = (T(1) - T(0)) / static_cast<T>(p.nx);
+  h.dy = (T(1) - T(0)) / static_cast<T>(p.ny);
+  h.conv_tol = static_cast<T>(1e-6f);
+  h.slope_type = p.slope_type;
+  h.niter = p.niter_riemann;
+  h.riemann_solver = p.riemann_solver;
+  return h;
+}
+
+struct Grid {
+  int nx, ny, isize, jsize;
+  int64_t pitch, plane;
+  int bc[4];  // xmin, xmax, ymin, ymax
+};
+
+static Grid make_grid(const e2d_params& p, const DevBuffers& b) {
+  Grid g;
+  g.nx = p.nx;
+  g.ny = p.ny;
+  g.isize = p.nx + 4;
+  g.jsize = p.ny + 4;
+  g.pitch = b.pitch;
+  g.plane = b.plane;
+  g.bc[0] = p.boundary_type_xmin;
+  g.bc[1] = p.boundary_type_xmax;
+  g.bc[2] = p.boundary_type_ymin;
+  g.bc[3] = p.boundary_type_ymax;
+  return g;
+}
+
+// Where the value of cell (i, j) comes from once boundaries are filled.
+// Non-corner ghost cells map to an interior cell (reflective: mirror with the
+// normal momentum negated; absorbing: nearest interior cell; other: periodic),
+// everything else (interior cells and the never-filled corners) maps to itself.
+struct Src {
+  int i, j;
+  int flip;  // variable whose sign flips (2 = rho*u, 3 = rho*v), or -1
+  bool ghost;
+};
+
+__device__ __forceinline__ Src ghost_source(const Grid& g, int i, int j) {
+  Src s{i, j, -1, false};
+  const bool xg = (i < 2) || (i >= g.nx + 2);
+  const bool yg = (j < 2) || (j >= g.ny + 2);
+  if (xg && !yg) {
+    s.ghost = true;
+    if (i < 2) {
+      const int bc = g.bc[0];
+      if (bc == 1) {
+        s.i = 3 - i;
+        s.flip = 2;
+      } else if (bc == 2) {
+        s.i = 2;
+      } else {
+        s.i = g.nx + i;
+      }
+    } else {
+      const int bc = g.bc[1];
+      if (bc == 1) {
+        s.i = 2 * g.nx + 3 - i;
+        s.flip = 2;
+      } else if (bc == 2) {
+        s.i = g.nx + 1;
+      } else {
+        s.i = i - g.nx;
+      }
+    }
+  } else if (yg && !xg) {
+    s.ghost = true;
+    if (j < 2) {
+      const int bc = g.bc[2];
+      if (bc == 1) {
+        s.j = 3 - j;
+        s.flip = 3;
+      } else if (bc == 2) {
+        s.j = 2;
+      } else {
+        s.j = g.ny + j;
+      }
+    } else {
+      const int bc = g.bc[3];
+      if (bc == 1) {
+        s.j = 2 * g.ny + 3 - j;
+        s.flip = 3;
+      } else if (bc == 2) {
+        s.j = g.ny + 1;
+      } else {
+        s.j = j - g.ny;
+      }
+    }
+  }
+  return s;
+}
+
+__device__ __forceinline__ int64_t at(const Grid& g, int i, int j, int v) {
+  return static_cast<int64_t>(v) * g.plane + static_cast<int64_t>(j) * g.pitch + i;
+}
+
+// ---------------------------------------------------------------------------
+// cross-lane helpers (wave64)
+template <class T>
+__device__ __forceinline__ T from_left(T x) {  // value of lane-1
+  return __shfl_up(x, 1, 64);
+}
+template <class T>
+__device__ __forceinline__ T from_right(T x) {  // value of lane+1
+  return __shfl_down(x, 1, 64);
+}
+template <class T>
+__device__ __forceinline__ T wave_max(T x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const T y = __shfl_xor(x, o, 64);
+    x = x > y ? x : y;
+  }
+  return x;
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(double* dst, double v) {
+  // IEEE order of non-negative doubles equals the order of their bit patterns
+  atomicMax(reinterpret_cast<unsigned long long*>(dst), static_cast<unsigned long long>(__double_as_longlong(v)));
+}
+
+// XCD-aware block order: hardware hands consecutive workgroups to the 8 XCDs
+// in turn; this gives each XCD a contiguous range of logical blocks so that
+// neighbouring strips/segments (which share halo rows and columns) meet in the
+// same L2.
+__device__ __forceinline__ int xcd_logical_block(int b, int nb) {
+  constexpr int kXcd = 8;
+  const int q = nb / kXcd, r = nb % kXcd;
+  const int xcd = b % kXcd, k = b / kXcd;
+  return (xcd < r) ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
+}
+
+// ---------------------------------------------------------------------------
+// fused step
+
+constexpr int kStripOut = 60;     // output columns per wavefront
+constexpr int kWavesPerBlock = 4;
+
+template <class P>
+struct StepArgs {
+  Hydro<typename P::T> hT;
+  Hydro<typename P::U> hU;
+  Grid g;
+  typename P::S* uin;   // read (interior, corners) and ghost write-back
+  typename P::S* uout;  // interior written
+  double* scal;
+  int dt_from_device, slot_cur, track_dt, slot_next;
+  double dt_host;
+  int nsegs;      // row segments per strip
+  int seg_rows;   // output rows per segment
+  int seg_groups; // ceil(nsegs / kWavesPerBlock)
+};
+
+// Load cell (ci, r) of the input, boundary-filled, as T; write ghost values back.
+template <class P>
+__device__ __forceinline__ void load_cell(const StepArgs<P>& a, int ci, bool col_ok, int r,
+                                          typename P::T u[4]) {
+  using T = typename P::T;
+  using S = typename P::S;
+  const Src s = ghost_source(a.g, ci, r);
+  S raw[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const S x = a.uin[at(a.g, s.i, s.j, v)];
+    raw[v] = (v == s.flip) ? -x : x;
+  }
+  if (s.ghost && col_ok) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) a.uin[at(a.g, ci, r, v)] = raw[v];
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) u[v] = static_cast<T>(raw[v]);
+}
+
+template <class P, int SOLVER>
+__global__ __launch_bounds__(256) void k_step_fused(const StepArgs<P> a) {
+  using T = typename P::T;
+  using U = typename P::U;
+  using S = typename P::S;
+  using M = typename P::M;
+
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int strip = lb / a.seg_groups;
+  const int seg = (lb % a.seg_groups) * kWavesPerBlock + wave;
+  if (seg >= a.nsegs) return;  // whole wavefront leaves: no barrier in this kernel
+
+  const Grid& g = a.g;
+  const int ci = strip * kStripOut + lane;  // column of this lane
+  const bool col_ok = ci < g.isize;
+  const int cl = col_ok ? ci : g.isize - 1;  // clamped column for loads
+  const bool out_lane = (lane >= 2) && (lane < 62) && (ci <= g.nx + 1);
+  const int j0 = 2 + seg * a.seg_rows;
+  const int j1 = min(j0 + a.seg_rows, g.ny + 2);
+
+  // dt of this step (serial_ref/m_run.f90:308, 384-385)
+  U dt;
+  double invdt_cur = 0.0;
+  if (a.dt_from_device) {
+    invdt_cur = a.scal[a.slot_cur];
+    dt = a.hU.cfl / static_cast<U>(invdt_cur);
+  } else {
+    dt = static_cast<U>(a.dt_host);
+  }
+  const U dtdxU = dt / a.hU.dx;
+  const U dtdyU = dt / a.hU.dy;
+  const T dtdx = static_cast<T>(dtdxU);
+  const T dtdy = static_cast<T>(dtdyU);
+  const Hydro<T>& h = a.hT;
+
+  T qa[4], qb[4], qc[4], u[4];
+  load_cell<P>(a, cl, col_ok, j0 - 2, u);
+  cons_to_prim<T, M>(h, u, qa);
+  load_cell<P>(a, cl, col_ok, j0 - 1, u);
+  cons_to_prim<T, M>(h, u, qb);
+
+  T ymax_prev[4];
+  U fy_prev[4] = {U(0), U(0), U(0), U(0)}, fx_prev[4] = {U(0), U(0), U(0), U(0)};
+  U fy_cur[4] = {U(0), U(0), U(0), U(0)}, fx_cur[4] = {U(0), U(0), U(0), U(0)};
+  U rate_max = U(0);
+
+  for (int r = j0 - 1; r <= j1; ++r) {
+    load_cell<P>(a, cl, col_ok, r + 1, u);
+    cons_to_prim<T, M>(h, u, qc);
+
+    // slopes and trace sources of row r
+    T qxm[4], qxp[4], dqx[4], dqy[4], s[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      qxm[v] = from_left(qb[v]);
+      qxp[v] = from_right(qb[v]);
+    }
+    cell_slopes(h, qb, qxp, qxm, qc, qa, dqx, dqy);
+    trace_sources<T, M>(h, qb, dqx, dqy, s);
+
+    // y-face between rows r-1 and r
+    if (r >= j0) {
+      T ymin[4], f[4];
+      face_state(h, qb, dqy, s, dtdy, false, ymin);
+      riemann_y<T, M, SOLVER>(h, ymax_prev, ymin, f);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) fy_cur[v] = static_cast<U>(f[v]) * dtdyU;
+    }
+    face_state(h, qb, dqy, s, dtdy, true, ymax_prev);
+
+    // x-face between columns ci-1 and ci, row r
+    if (r >= j0 && r < j1) {
+      T xmin[4], xmax[4], xl[4], gd[4], f[4];
+      face_state(h, qb, dqx, s, dtdx, false, xmin);
+      face_state(h, qb, dqx, s, dtdx, true, xmax);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) xl[v] = from_left(xmax[v]);
+      riemann<T, M, SOLVER>(h, xl, xmin, gd, f);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) fx_cur[v] = static_cast<U>(f[v]) * dtdxU;
+    }
+
+    // conservative update of row r-1 (updateHydro, serial_ref/m_run.f90:504-513)
+    if (r >= j0 + 1) {
+      U fxr[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) fxr[v] = from_right(fx_prev[v]);
+      if (out_lane) {
+        const int jr = r - 1;
+        U un[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const U uo = static_cast<U>(a.uin[at(g, ci, jr, v)]);
+          const U flux_tot = (fy_prev[v] - fy_cur[v]) + (fx_prev[v] - fxr[v]);
+          const S sn = static_cast<S>(uo + flux_tot);
+          a.uout[at(g, ci, jr, v)] = sn;
+          un[v] = static_cast<U>(sn);
+        }
+        // CFL rate of the new state over the compute_dt window [2, n] (0-based)
+        if (a.track_dt && ci <= g.nx && jr <= g.ny) {
+          const U rate = cfl_rate<U, ExactMath>(a.hU, un);
+          rate_max = rate_max > rate ? rate_max : rate;
+        }
+      }
+    }
+
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      qa[v] = qb[v];
+      qb[v] = qc[v];
+      fy_prev[v] = fy_cur[v];
+      fx_prev[v] = fx_cur[v];
+    }
+  }
+
+  if (a.track_dt) {
+    double m = static_cast<double>(rate_max);
+    m = m > invdt_cur ? m : invdt_cur;  // running maximum carries over
+    m = wave_max(m);
+    if (lane == 0) atomic_max_nonneg(a.scal + a.slot_next, m);
+  }
+}
+
+template <class P>
+static hipError_t fused_step(const e2d_params& p, const DevBuffers& b, const StepCfg& c, hipStream_t st) {
+  StepArgs<P> a;
+  a.hT = make_hydro<typename P::T>(p);
+  a.hU = make_hydro<typename P::U>(p);
+  a.g = make_grid(p, b);
+  a.uin = static_cast<typename P::S*>(b.u[c.buf_in]);
+  a.uout = static_cast<typename P::S*>(b.u[c.buf_out]);
+  a.scal = b.scal;
+  a.dt_from_device = c.dt_from_device ? 1 : 0;
+  a.slot_cur = c.slot_cur;
+  a.track_dt = c.track_dt ? 1 : 0;
+  a.slot_next = c.slot_next;
+  a.dt_host = c.dt_host;
+  a.seg_rows = 16;
+  a.nsegs = (p.ny + a.seg_rows - 1) / a.seg_rows;
+  a.seg_groups = (a.nsegs + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int nstrips = (p.nx + kStripOut - 1) / kStripOut;
+  const int nblocks = nstrips * a.seg_groups;
+  if (p.riemann_solver == E2D_RIEMANN_HLLC)
+    hipLaunchKernelGGL((k_step_fused<P, 2>), dim3(nblocks), dim3(64 * kWavesPerBlock), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_step_fused<P, 0>), dim3(nblocks), dim3(64 * kWavesPerBlock), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_fused_step(const e2d_params& p, const DevBuffers& b, const StepCfg& c, hipStream_t s) {
+  switch (p.precision) {
+    case E2D_PREC_FP64: return fused_step<P64>(p, b, c, s);
+    case E2D_PREC_FP32: return fused_step<P32>(p, b, c, s);
+    default: return fused_step<PMix>(p, b, c, s);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// boundaries (make_boundaries): one thread per non-corner ghost cell
+
+template <class S>
+__global__ __launch_bounds__(256) void k_boundaries(const Grid g, S* u) {
+  const int nxg = 4 * g.ny;  // x-face ghosts: 2 columns each side, interior rows
+  const int nyg = 4 * g.nx;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j;
+  if (t < nxg) {
+    const int k = t % 4;
+    j = 2 + t / 4;
+    i = (k < 2) ? k : g.nx + k;  // 0,1 | nx+2, nx+3
+  } else if (t < nxg + nyg) {
+    const int t2 = t - nxg;
+    const int k = t2 % 4;
+    i = 2 + t2 / 4;
+    j = (k < 2) ? k : g.ny + k;
+  } else {
+    return;
+  }
+  const Src s = ghost_source(g, i, j);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    S x = u[at(g, s.i, s.j, v)];
+    if (v == s.flip) x = -x;
+    u[at(g, i, j, v)] = x;
+  }
+}
+
+template <class P>
+static hipError_t boundaries(const e2d_params& p, const DevBuffers& b, int buf, hipStream_t st) {
+  const Grid g = make_grid(p, b);
+  const int n = 4 * (p.nx + p.ny);
+  hipLaunchKernelGGL(k_boundaries<typename P::S>, dim3((n + 255) / 256), dim3(256), 0, st, g,
+                     static_cast<typename P::S*>(b.u[buf]));
+  return hipGetLastError();
+}
+
+hipError_t launch_boundaries(const e2d_params& p, const DevBuffers& b, int buf, hipStream_t s) {
+  switch (p.precision) {
+    case E2D_PREC_FP64: return boundaries<P64>(p, b, buf, s);
+    case E2D_PREC_FP32: return boundaries<P32>(p, b, buf, s);
+    default: return boundaries<PMix>(p, b, buf, s);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// compute_dt reduction: max CFL rate over [2, nx] x [2, ny] (0-based)
+
+template <class P>
+__global__ __launch_bounds__(256) void k_max_invdt(const Hydro<typename P::U> h, const Grid g,
+                                                    const typename P::S* __restrict__ u, double* dst) {
+  using U = typename P::U;
+  __shared__ double red[4];
+  const int w = g.nx - 1, hgt = g.ny - 1;  // window extent
+  const int64_t n = static_cast<int64_t>(w) * hgt;
+  U m = U(0);
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int i = 2 + static_cast<int>(t % w);
+    const int j = 2 + static_cast<int>(t / w);
+    U c[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) c[v] = static_cast<U>(u[at(g, i, j, v)]);
+    const U r = cfl_rate<U, ExactMath>(h, c);
+    m = m > r ? m : r;
+  }
+  double md = wave_max(static_cast<double>(m));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = md;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = red[0];
+    for (int k = 1; k < 4; ++k) x = x > red[k] ? x : red[k];
+    atomic_max_nonneg(dst, x);
+  }
+}
+
+template <class P>
+static hipError_t max_invdt(const e2d_params& p, const DevBuffers& b, int buf, int slot, hipStream_t st) {
+  const Grid g = make_grid(p, b);
+  const int64_t n = static_cast<int64_t>(p.nx - 1) * (p.ny - 1);
+  int nb = static_cast<int>((n + 255) / 256);
+  if (nb > 2048) nb = 2048;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(k_max_invdt<P>, dim3(nb), dim3(256), 0, st, make_hydro<typename P::U>(p), g,
+                     static_cast<const typename P::S*>(b.u[buf]), b.scal + slot);
+  return hipGetLastError();
+}
+
+hipError_t launch_max_invdt(const e2d_params& p, const DevBuffers& b, int buf, int slot, hipStream_t s) {
+  switch (p.precision) {
+    case E2D_PREC_FP64: return max_invdt<P64>(p, b, buf, slot, s);
+    case E2D_PREC_FP32: return max_invdt<P32>(p, b, buf, slot, s);
+    default: return max_invdt<PMix>(p, b, buf, slot, s);
+  }
+}
+
+__global__ void k_fill(double* dst, int n, double v) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = v;
+}
+
+hipError_t launch_fill(double* dst, int n, double v, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill, dim3((n + 63) / 64), dim3(64), 0, s, dst, n, v);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// materialised step (implementationVersion 1)
+
+template <class P>
+struct MatArgs {
+  Hydro<typename P::T> hT;
+  Hydro<typename P::U> hU;
+  Grid g;
+  const typename P::S* uin;
+  typename P::S* uout;
+  typename P::T *q, *dqx, *dqy, *qxm, *qxp, *qym, *qyp;
+  typename P::U *fx, *fy;
+  const double* scal;
+  int dt_from_device, slot_cur;
+  double dt_host;
+};
+
+template <class P>
+__device__ __forceinline__ void step_dt(const MatArgs<P>& a, typename P::U& dtdx, typename P::U& dtdy) {
+  using U = typename P::U;
+  const U dt = a.dt_from_device ? a.hU.cfl / static_cast<U>(a.scal[a.slot_cur]) : static_cast<U>(a.dt_host);
+  dtdx = dt / a.hU.dx;
+  dtdy = dt / a.hU.dy;
+}
+
+// convertToPrimitives over all cells (serial_ref/m_run.f90:338-354)
+template <class P>
+__global__ __launch_bounds__(256) void k_prim(const MatArgs<P> a) {
+  using T = typename P::T;
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (i >= a.g.isize || j >= a.g.jsize) return;
+  T u[4], q[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) u[v] = static_cast<T>(a.uin[at(a.g, i, j, v)]);
+  cons_to_prim<T, typename P::M>(a.hT, u, q);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) a.q[at(a.g, i, j, v)] = q[v];
+}
+
+// slopes of cells [1, size-2]^2
+template <class P>
+__global__ __launch_bounds__(256) void k_slopes(const MatArgs<P> a) {
+  using T = typename P::T;
+  const int i = 1 + blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (i > a.g.isize - 2 || j > a.g.jsize - 2) return;
+  T q[4], xp[4], xm[4], yp[4], ym[4], dx[4], dy[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    q[v] = a.q[at(a.g, i, j, v)];
+    xp[v] = a.q[at(a.g, i + 1, j, v)];
+    xm[v] = a.q[at(a.g, i - 1, j, v)];
+    yp[v] = a.q[at(a.g, i, j + 1, v)];
+    ym[v] = a.q[at(a.g, i, j - 1, v)];
+  }
+  cell_slopes(a.hT, q, xp, xm, yp, ym, dx, dy);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    a.dqx[at(a.g, i, j, v)] = dx[v];
+    a.dqy[at(a.g, i, j, v)] = dy[v];
+  }
+}
+
+// the four face states of cells [1, size-2]^2
+template <class P>
+__global__ __launch_bounds__(256) void k_trace(const MatArgs<P> a) {
+  using T = typename P::T;
+  using U = typename P::U;
+  const int i = 1 + blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (i > a.g.isize - 2 || j > a.g.jsize - 2) return;
+  U dtdxU, dtdyU;
+  step_dt(a, dtdxU, dtdyU);
+  const T dtdx = static_cast<T>(dtdxU), dtdy = static_cast<T>(dtdyU);
+  T q[4], dx[4], dy[4], s[4], f[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    q[v] = a.q[at(a.g, i, j, v)];
+    dx[v] = a.dqx[at(a.g, i, j, v)];
+    dy[v] = a.dqy[at(a.g, i, j, v)];
+  }
+  trace_sources<T, typename P::M>(a.hT, q, dx, dy, s);
+  face_state(a.hT, q, dx, s, dtdx, true, f);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) a.qxm[at(a.g, i, j, v)] = f[v];
+  face_state(a.hT, q, dx, s, dtdx, false, f);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) a.qxp[at(a.g, i, j, v)] = f[v];
+  face_state(a.hT, q, dy, s, dtdy, true, f);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) a.qym[at(a.g, i, j, v)] = f[v];
+  face_state(a.hT, q, dy, s, dtdy, false, f);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) a.qyp[at(a.g, i, j, v)] = f[v];
+}
+
+// fluxes of the faces of computeAndStoreFluxes, (i, j) in [2, size-2]^2
+// (serial_ref/m_run.f90:389-480)
+template <class P, int SOLVER>
+__global__ __launch_bounds__(256) void k_flux(const MatArgs<P> a) {
+  using T = typename P::T;
+  using U = typename P::U;
+  const int i = 2 + blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = 2 + blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (i > a.g.isize - 2 || j > a.g.jsize - 2) return;
+  U dtdx, dtdy;
+  step_dt(a, dtdx, dtdy);
+  T l[4], r[4], gd[4], f[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    l[v] = a.qxm[at(a.g, i - 1, j, v)];
+    r[v] = a.qxp[at(a.g, i, j, v)];
+  }
+  riemann<T, typename P::M, SOLVER>(a.hT, l, r, gd, f);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) a.fx[at(a.g, i, j, v)] = static_cast<U>(f[v]) * dtdx;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    l[v] = a.qym[at(a.g, i, j - 1, v)];
+    r[v] = a.qyp[at(a.g, i, j, v)];
+  }
+  riemann_y<T, typename P::M, SOLVER>(a.hT, l, r, f);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) a.fy[at(a.g, i, j, v)] = static_cast<U>(f[v]) * dtdy;
+}
+
+// updateHydro over the interior (serial_ref/m_run.f90:504-513)
+template <class P>
+__global__ __launch_bounds__(256) void k_update(const MatArgs<P> a) {
+  using U = typename P::U;
+  using S = typename P::S;
+  const int i = 2 + blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = 2 + blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (i > a.g.nx + 1 || j > a.g.ny + 1) return;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    U flux_tot = a.fx[at(a.g, i, j, v)] - a.fx[at(a.g, i + 1, j, v)];
+    flux_tot = a.fy[at(a.g, i, j, v)] - a.fy[at(a.g, i, j + 1, v)] + flux_tot;
+    a.uout[at(a.g, i, j, v)] = static_cast<S>(static_cast<U>(a.uin[at(a.g, i, j, v)]) + flux_tot);
+  }
+}
+
+template <class P>
+static hipError_t materialised_step(const e2d_params& p, const DevBuffers& b, const StepCfg& c,
+                                    hipStream_t st) {
+  using T = typename P::T;
+  using U = typename P::U;
+  MatArgs<P> a;
+  a.hT = make_hydro<T>(p);
+  a.hU = make_hydro<U>(p);
+  a.g = make_grid(p, b);
+  a.uin = static_cast<const typename P::S*>(b.u[c.buf_in]);
+  a.uout = static_cast<typename P::S*>(b.u[c.buf_out]);
+  a.q = static_cast<T*>(b.stage[E2D_STAGE_Q]);
+  a.fx = static_cast<U*>(b.stage[E2D_STAGE_FX]);
+  a.fy = static_cast<U*>(b.stage[E2D_STAGE_FY]);
+  a.dqx = static_cast<T*>(b.stage[E2D_STAGE_DQX]);
+  a.dqy = static_cast<T*>(b.stage[E2D_STAGE_DQY]);
+  a.qxm = static_cast<T*>(b.stage[E2D_STAGE_QXM]);
+  a.qxp = static_cast<T*>(b.stage[E2D_STAGE_QXP]);
+  a.qym = static_cast<T*>(b.stage[E2D_STAGE_QYM]);
+  a.qyp = static_cast<T*>(b.stage[E2D_STAGE_QYP]);
+  a.scal = b.scal;
+  a.dt_from_device = c.dt_from_device ? 1 : 0;
+  a.slot_cur = c.slot_cur;
+  a.dt_host = c.dt_host;
+  const int S_ = p.nx + 4, J_ = p.ny + 4;
+  hipError_t e = boundaries<P>(p, b, c.buf_in, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_prim<P>, dim3((S_ + 63) / 64, (J_ + 3) / 4), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_slopes<P>, dim3((S_ - 2 + 63) / 64, (J_ - 2 + 3) / 4), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_trace<P>, dim3((S_ - 2 + 63) / 64, (J_ - 2 + 3) / 4), dim3(256), 0, st, a);
+  const dim3 fgrid((S_ - 3 + 63) / 64, (J_ - 3 + 3) / 4);
+  if (p.riemann_solver == E2D_RIEMANN_HLLC)
+    hipLaunchKernelGGL((k_flux<P, 2>), fgrid, dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_flux<P, 0>), fgrid, dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_update<P>, dim3((p.nx + 63) / 64, (p.ny + 3) / 4), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_materialised_step(const e2d_params& p, const DevBuffers& b, const StepCfg& c,
+                                    hipStream_t s) {
+  switch (p.precision) {
+    case E2D_PREC_FP64: return materialised_step<P64>(p, b, c, s);
+    case E2D_PREC_FP32: return materialised_step<P32>(p, b, c, s);
+    default: return materialised_step<PMix>(p, b, c, s);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// initial conditions (init_implode, serial_ref/m_run.f90:524-555; diag; bumps)
+
+struct Bumps {
+  double v[16 * 5];  // cx, cy, amp_rho, amp_p, sigma
+};
+
+template <class P>
+__global__ __launch_bounds__(256) void k_init(const Grid g, typename P::S* u, typename P::S* u2,
+                                              int problem, typename P::U gamma0, Bumps bumps) {
+  using U = typename P::U;
+  using S = typename P::S;
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (i >= g.isize || j >= g.jsize) return;
+  U rho, e;
+  if (problem == E2D_PROBLEM_IMPLODE) {
+    // tmp in default real (FP32) whatever the run's precision
+    const float tmp = 1.0f * static_cast<float>(i - 2) / static_cast<float>(g.nx) +
+                      1.0f * static_cast<float>(j - 2) / static_cast<float>(g.ny);
+    if (static_cast<U>(tmp) > U(0.5)) {
+      rho = U(1.0);
+      e = U(1.0) / (gamma0 - U(1.0));
+    } else {
+      rho = U(0.125);
+      e = static_cast<U>(0.14f) / (gamma0 - U(1.0));
+    }
+  } else {
+    const U x = static_cast<U>(i - 2) / static_cast<U>(g.nx);
+    const U y = static_cast<U>(j - 2) / static_cast<U>(g.ny);
+    U pr;
+    if (x + y < U(1)) {
+      rho = U(1);
+      pr = U(1);
+    } else {
+      rho = U(0.125);
+      pr = U(0.1);
+    }
+    if (problem == E2D_PROBLEM_BUMPS) {
+      const double xc = (static_cast<double>(i - 2) + 0.5) / g.nx;
+      const double yc = (static_cast<double>(j - 2) + 0.5) / g.ny;
+      double dr = 0.0, dp = 0.0;
+      for (int k = 0; k < 16; ++k) {
+        const double* bk = bumps.v + 5 * k;
+        const double ddx = xc - bk[0], ddy = yc - bk[1], sg = bk[4];
+        const double w = exp(-(ddx * ddx + ddy * ddy) / (2.0 * sg * sg));
+        dr += bk[2] * w;
+        dp += bk[3] * w;
+      }
+      rho = rho + static_cast<U>(dr);
+      pr = pr + static_cast<U>(dp);
+    }
+    e = pr / (gamma0 - U(1));
+  }
+  const S vals[4] = {static_cast<S>(rho), static_cast<S>(e), S(0), S(0)};
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    u[at(g, i, j, v)] = vals[v];
+    u2[at(g, i, j, v)] = S(0);
+  }
+}
+
+template <class P>
+static hipError_t init_problem(const e2d_params& p, const DevBuffers& b, int problem, const double* bumps,
+                               hipStream_t st) {
+  const Grid g = make_grid(p, b);
+  Bumps bb{};
+  if (bumps)
+    for (int k = 0; k < 16 * 5; ++k) bb.v[k] = bumps[k];
+  hipLaunchKernelGGL(k_init<P>, dim3((g.isize + 63) / 64, (g.jsize + 3) / 4), dim3(256), 0, st, g,
+                     static_cast<typename P::S*>(b.u[0]), static_cast<typename P::S*>(b.u[1]), problem,
+                     static_cast<typename P::U>(p.gamma0), bb);
+  return hipGetLastError();
+}
+
+hipError_t launch_init_problem(const e2d_params& p, const DevBuffers& b, int problem, const double* bumps,
+                               hipStream_t s) {
+  switch (p.precision) {
+    case E2D_PREC_FP64: return init_problem<P64>(p, b, problem, bumps, s);
+    case E2D_PREC_FP32: return init_problem<P32>(p, b, problem, bumps, s);
+    default: return init_problem<PMix>(p, b, problem, bumps, s);
+  }
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void k_zero(const Grid g, S* u) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (i >= g.isize || j >= g.jsize) return;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) u[at(g, i, j, v)] = S(0);
+}
+
+hipError_t launch_zero_state(const e2d_params& p, const DevBuffers& b, int buf, hipStream_t s) {
+  const Grid g = make_grid(p, b);
+  const dim3 grid((g.isize + 63) / 64, (g.jsize + 3) / 4);
+  if (state_elem_bytes(p.precision) == 8)
+    hipLaunchKernelGGL(k_zero<double>, grid, dim3(256), 0, s, g, static_cast<double*>(b.u[buf]));
+  else
+    hipLaunchKernelGGL(k_zero<float>, grid, dim3(256), 0, s, g, static_cast<float*>(b.u[buf]));
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// deterministic FP64 sums over the interior: block k reduces rows k, k+B, ...
+// in a fixed order, then one block adds the partials in block order.
+
+template <class S>
+__global__ __launch_bounds__(256) void k_sums_partial(const Grid g, const S* __restrict__ u, double* partial) {
+  __shared__ double red[4][256];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int j = 2 + blockIdx.x; j <= g.ny + 1; j += gridDim.x)
+    for (int i = 2 + threadIdx.x; i <= g.nx + 1; i += blockDim.x)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[v] += static_cast<double>(u[at(g, i, j, v)]);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) red[v][threadIdx.x] = acc[v];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) red[v][threadIdx.x] += red[v][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) partial[blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_sums_final(const double* partial, int nb, double* out) {
+  if (threadIdx.x < 4) {
+    double s = 0.0;
+    for (int k = 0; k < nb; ++k) s += partial[k * 4 + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
+
+hipError_t launch_sums(const e2d_params& p, const DevBuffers& b, int buf, double* d_out4, hipStream_t s) {
+  const Grid g = make_grid(p, b);
+  if (state_elem_bytes(p.precision) == 8)
+    hipLaunchKernelGGL(k_sums_partial<double>, dim3(kSumBlocks), dim3(256), 0, s, g,
+                       static_cast<const double*>(b.u[buf]), b.partial);
+  else
+    hipLaunchKernelGGL(k_sums_partial<float>, dim3(kSumBlocks), dim3(256), 0, s, g,
+                       static_cast<const float*>(b.u[buf]), b.partial);
+  hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(64), 0, s, b.partial, kSumBlocks, d_out4);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// host layout <-> pitched device layout
+
+template <class S>
+__global__ __launch_bounds__(256) void k_pack(const Grid g, const double* __restrict__ src, S* dst) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (i >= g.isize || j >= g.jsize) return;
+  const int64_t hp = static_cast<int64_t>(g.isize) * g.jsize;
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    dst[at(g, i, j, v)] = static_cast<S>(src[v * hp + static_cast<int64_t>(j) * g.isize + i]);
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void k_unpack(const Grid g, const S* __restrict__ src, double* dst) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (i >= g.isize || j >= g.jsize) return;
+  const int64_t hp = static_cast<int64_t>(g.isize) * g.jsize;
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    dst[v * hp + static_cast<int64_t>(j) * g.isize + i] = static_cast<double>(src[at(g, i, j, v)]);
+}
+
+hipError_t launch_pack(const e2d_params& p, const DevBuffers& b, int buf, const double* d_src, hipStream_t s) {
+  const Grid g = make_grid(p, b);
+  const dim3 grid((g.isize + 63) / 64, (g.jsize + 3) / 4);
+  if (state_elem_bytes(p.precision) == 8)
+    hipLaunchKernelGGL(k_pack<double>, grid, dim3(256), 0, s, g, d_src, static_cast<double*>(b.u[buf]));
+  else
+    hipLaunchKernelGGL(k_pack<float>, grid, dim3(256), 0, s, g, d_src, static_cast<float*>(b.u[buf]));
+  return hipGetLastError();
+}
+
+hipError_t launch_unpack(const e2d_params& p, const DevBuffers& b, const void* d_state, int elem_bytes,
+                         double* d_dst, hipStream_t s) {
+  const Grid g = make_grid(p, b);
+  const dim3 grid((g.isize + 63) / 64, (g.jsize + 3) / 4);
+  if (elem_bytes == 8)
+    hipLaunchKernelGGL(k_unpack<double>, grid, dim3(256), 0, s, g, static_cast<const double*>(d_state), d_dst);
+  else
+    hipLaunchKernelGGL(k_unpack<float>, grid, dim3(256), 0, s, g, static_cast<const float*>(d_state), d_dst);
+  return hipGetLastError();
+}
+
+size_t state_elem_bytes(int precision) { return precision == E2D_PREC_FP64 ? 8 : 4; }
+
+size_t stage_elem_bytes(int precision, int stage) {
+  if (precision == E2D_PREC_FP64) return 8;
+  if (precision == E2D_PREC_FP32) return 4;
+  return (stage == E2D_STAGE_FX || stage == E2D_STAGE_FY) ? 8 : 4;  // mixed: fluxes kept in FP64
+}
+
+// ---------------------------------------------------------------------------
+// pointwise numerics (known-answer tests of the device routines)
+
+template <class P>
+__global__ __launch_bounds__(256) void k_pointwise(const Hydro<typename P::T> h, int func, int n,
+                                                    const double* __restrict__ x, double* __restrict__ y) {
+  using T = typename P::T;
+  using M = typename P::M;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  T a[20], o[8], g[4];
+  switch (func) {
+    case E2D_FN_PRIM: {
+      for (int m = 0; m < 4; ++m) a[m] = static_cast<T>(x[4 * k + m]);
+      cons_to_prim<T, M>(h, a, o);
+      const T c = sound_speed<T, M>(h, o);
+      for (int m = 0; m < 4; ++m) y[5 * k + m] = o[m];
+      y[5 * k + 4] = c;
+    } break;
+    case E2D_FN_SLOPE: {
+      for (int m = 0; m < 20; ++m) a[m] = static_cast<T>(x[20 * k + m]);
+      cell_slopes(h, a, a + 4, a + 8, a + 12, a + 16, o, o + 4);
+      for (int m = 0; m < 8; ++m) y[8 * k + m] = o[m];
+    } break;
+    case E2D_FN_TRACE: {
+      for (int m = 0; m < 14; ++m) a[m] = static_cast<T>(x[15 * k + m]);
+      const int face = static_cast<int>(x[15 * k + 14]);
+      T s[4];
+      trace_sources<T, M>(h, a, a + 4, a + 8, s);
+      const bool xdir = face < 2;
+      face_state(h, a, xdir ? a + 4 : a + 8, s, xdir ? a[12] : a[13], (face & 1) == 1, o);
+      for (int m = 0; m < 4; ++m) y[4 * k + m] = o[m];
+    } break;
+    case E2D_FN_RIEMANN:
+    case E2D_FN_HLLC: {
+      for (int m = 0; m < 8; ++m) a[m] = static_cast<T>(x[8 * k + m]);
+      if (func == E2D_FN_RIEMANN)
+        riemann_two_shock<T, M>(h, a, a + 4, g, o);
+      else
+        riemann_hllc<T, M>(h, a, a + 4, g, o);
+      for (int m = 0; m < 4; ++m) y[8 * k + m] = g[m];
+      for (int m = 0; m < 4; ++m) y[8 * k + 4 + m] = o[m];
+    } break;
+    case E2D_FN_CMPFLX: {
+      for (int m = 0; m < 4; ++m) a[m] = static_cast<T>(x[4 * k + m]);
+      godunov_flux(h, a, o);
+      for (int m = 0; m < 4; ++m) y[4 * k + m] = o[m];
+    } break;
+    case E2D_FN_EOS: {
+      const T rho = static_cast<T>(x[2 * k]), eint = static_cast<T>(x[2 * k + 1]);
+      const T p = fmax2((h.gamma0 - T(1)) * rho * eint, rho * h.smallp);
+      const T c = M::sqrt(M::div(h.gamma0 * p, rho));
+      y[2 * k] = p;
+      y[2 * k + 1] = c;
+    } break;
+    default: break;
+  }
+}
+
+hipError_t launch_pointwise(const e2d_params& p, int func, int n, const double* d_x, double* d_y, hipStream_t s) {
+  const dim3 grid((n + 255) / 256);
+  switch (p.precision) {
+    case E2D_PREC_FP64:
+      hipLaunchKernelGGL(k_pointwise<P64>, grid, dim3(256), 0, s, make_hydro<double>(p), func, n, d_x, d_y);
+      break;
+    case E2D_PREC_FP32:
+      hipLaunchKernelGGL(k_pointwise<P32>, grid, dim3(256), 0, s, make_hydro<float>(p), func, n, d_x, d_y);
+      break;
+    default:
+      hipLaunchKernelGGL(k_pointwise<PMix>, grid, dim3(256), 0, s, make_hydro<float>(p), func, n, d_x, d_y);
+      break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace e2d

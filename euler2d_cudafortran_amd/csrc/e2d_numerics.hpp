@@ -1,0 +1,306 @@
+// Pointwise numerics of the euler2d hot path, for gfx950.
+//
+// Every routine keeps the reference's operation order (pkestene/euler2d_cudaFortran,
+// serial_ref/m_utils.f90) so that, built with -ffp-contract=off and the exact
+// math policy, a result is bit-identical to the reference's for the same inputs.
+// Reassociations that are exact are used freely: multiplication by +-1 becomes a
+// select of the sign, loop-invariant constants are hoisted, and the four face
+// states of a cell share one evaluation of the trace source terms.
+//
+// Variable order inside a 4-vector follows the reference's state planes:
+// [0] = rho, [1] = p (E for conservative vectors), [2] = u, [3] = v
+// (serial_ref/m_constants.f90:11-14).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace e2d {
+
+// Per-run constants in the arithmetic type T (serial_ref/m_parameters.f90:10-36,
+// derived as in serial_ref/m_run.f90:111-124).
+template <class T>
+struct Hydro {
+  T gamma0, gamma6, cfl, smallr, smallc, smallp, smallpp;
+  T entho;     // 1/(gamma0-1), evaluated as the reference evaluates it in cmpflx
+  T dx, dy;
+  T conv_tol;  // Newton stop threshold: the FP32 literal 1e-6 widened to T (m_utils.f90:396)
+  int slope_type, niter, riemann_solver;
+};
+
+// Division / square root policy.  ExactMath keeps IEEE correctly rounded
+// operations (bitwise parity).  FastMath (float only) uses the native
+// reciprocal and square root (about 1 ulp), for the mixed-precision variant.
+struct ExactMath {
+  static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+  static __device__ __forceinline__ float div(float a, float b) { return a / b; }
+  static __device__ __forceinline__ double sqrt(double a) { return __builtin_sqrt(a); }
+  static __device__ __forceinline__ float sqrt(float a) { return __builtin_sqrtf(a); }
+};
+struct FastMath {
+  static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+  static __device__ __forceinline__ float sqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
+};
+
+// Fortran max/min as the reference compiler lowers them (first operand wins
+// only when strictly greater / smaller; this fixes the sign of zero results).
+template <class T>
+__device__ __forceinline__ T fmax2(T a, T b) { return a > b ? a : b; }
+template <class T>
+__device__ __forceinline__ T fmin2(T a, T b) { return a < b ? a : b; }
+template <class T>
+__device__ __forceinline__ T fabs_(T a) { return __builtin_fabs(a); }
+template <>
+__device__ __forceinline__ float fabs_<float>(float a) { return __builtin_fabsf(a); }
+// s * x for s = +-1, exactly
+template <class T>
+__device__ __forceinline__ T signed_by(bool positive, T x) { return positive ? x : -x; }
+
+// Conservative -> primitive (computePrimitives + eos, m_utils.f90:35-64, 15-29).
+template <class T, class M>
+__device__ __forceinline__ void cons_to_prim(const Hydro<T>& h, const T u[4], T q[4]) {
+  const T rho = fmax2(u[0], h.smallr);
+  const T vx = M::div(u[2], rho);
+  const T vy = M::div(u[3], rho);
+  const T kinetic = T(0.5) * (vx * vx + vy * vy);
+  const T eint = M::div(u[1], rho) - kinetic;
+  q[0] = rho;
+  q[1] = fmax2((h.gamma0 - T(1)) * rho * eint, rho * h.smallp);
+  q[2] = vx;
+  q[3] = vy;
+}
+
+// Sound speed of a primitive state (second half of eos).
+template <class T, class M>
+__device__ __forceinline__ T sound_speed(const Hydro<T>& h, const T q[4]) {
+  return M::sqrt(M::div(h.gamma0 * q[1], q[0]));
+}
+
+// Local CFL rate (c+|u|)/dx + (c+|v|)/dy of a conservative state
+// (compute_dt body, serial_ref/m_run.f90:278-281).
+template <class T, class M>
+__device__ __forceinline__ T cfl_rate(const Hydro<T>& h, const T u[4]) {
+  T q[4];
+  cons_to_prim<T, M>(h, u, q);
+  const T c = sound_speed<T, M>(h, q);
+  const T ax = c + fabs_(q[2]);
+  const T ay = c + fabs_(q[3]);
+  return M::div(ax, h.dx) + M::div(ay, h.dy);
+}
+
+// Limited slope of one variable along one direction (slope_unsplit_hydro_2d,
+// m_utils.f90:298-312): minmod for slope_type 1, MC for 2.
+template <class T>
+__device__ __forceinline__ T limited_slope(T st, T qm, T q0, T qp) {
+  const T dl = st * (q0 - qm);
+  const T dr = st * (qp - q0);
+  const T dc = T(0.5) * (qp - qm);
+  T lim = fmin2(fabs_(dl), fabs_(dr));
+  if ((dl * dr) <= T(0)) lim = T(0);
+  return signed_by(dc >= T(0), fmin2(lim, fabs_(dc)));
+}
+
+// Slopes of a cell in both directions.  slope_type 0 and values other than
+// 1/2 give zero slopes (first order), m_utils.f90:289-337.
+template <class T>
+__device__ __forceinline__ void cell_slopes(const Hydro<T>& h, const T q[4], const T qxp[4],
+                                            const T qxm[4], const T qyp[4], const T qym[4],
+                                            T dqx[4], T dqy[4]) {
+  const bool limited = (h.slope_type == 1 || h.slope_type == 2);
+  const T st = T(h.slope_type);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const T sx = limited_slope(st, qxm[v], q[v], qxp[v]);
+    const T sy = limited_slope(st, qym[v], q[v], qyp[v]);
+    dqx[v] = limited ? sx : T(0);
+    dqy[v] = limited ? sy : T(0);
+  }
+}
+
+// Hancock predictor source terms of a cell (trace_unsplit_2d_along_dir,
+// m_utils.f90:213-216): s = (sr0, sp0, su0, sv0).
+template <class T, class M>
+__device__ __forceinline__ void trace_sources(const Hydro<T>& h, const T q[4], const T dqx[4],
+                                              const T dqy[4], T s[4]) {
+  const T r = q[0], p = q[1], u = q[2], v = q[3];
+  const T div_vel = dqx[2] + dqy[3];
+  s[0] = -u * dqx[0] - v * dqy[0] - div_vel * r;
+  s[1] = -u * dqx[1] - v * dqy[1] - div_vel * h.gamma0 * p;
+  s[2] = -u * dqx[2] - v * dqy[2] - M::div(dqx[1], r);
+  s[3] = -u * dqx[3] - v * dqy[3] - M::div(dqy[1], r);
+}
+
+// Face state of a cell: q + side*dq/2 + s*dt_dir/2 with only the density
+// floored (m_utils.f90:218-252).  side = +1 for the XMAX/YMAX face, -1 for
+// XMIN/YMIN; dq and dtd are those of the face's direction.
+template <class T>
+__device__ __forceinline__ void face_state(const Hydro<T>& h, const T q[4], const T dq[4],
+                                           const T s[4], T dtd, bool plus_side, T f[4]) {
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const T half_dq = T(0.5) * dq[v];
+    const T edge = plus_side ? q[v] + half_dq : q[v] - half_dq;
+    f[v] = edge + s[v] * dtd * T(0.5);
+  }
+  f[0] = fmax2(h.smallr, f[0]);
+}
+
+// Flux of a Godunov state (cmpflx, m_utils.f90:498-526).
+template <class T>
+__device__ __forceinline__ void godunov_flux(const Hydro<T>& h, const T g[4], T f[4]) {
+  const T mass = g[0] * g[2];
+  f[0] = mass;
+  f[2] = mass * g[2] + g[1];
+  f[3] = mass * g[3];
+  const T ekin = T(0.5) * g[0] * (g[2] * g[2] + g[3] * g[3]);
+  const T etot = g[1] * h.entho + ekin;
+  f[1] = g[2] * (etot + g[1]);
+}
+
+// Approximate two-shock Riemann solver (riemann_2d, m_utils.f90:349-492):
+// Newton iterations on p*, star state, sampling at x/t = 0, transverse
+// velocity upwinded by the contact.  Writes the Godunov state and its flux.
+template <class T, class M>
+__device__ __forceinline__ void riemann_two_shock(const Hydro<T>& h, const T ql[4], const T qr[4],
+                                                  T g[4], T f[4]) {
+  const T rl = fmax2(ql[0], h.smallr);
+  const T ul = ql[2];
+  const T pl = fmax2(ql[1], rl * h.smallp);
+  const T rr = fmax2(qr[0], h.smallr);
+  const T ur = qr[2];
+  const T pr = fmax2(qr[1], rr * h.smallp);
+
+  // Lagrangian sound speeds squared and first guess
+  const T cl = h.gamma0 * pl * rl;
+  const T cr = h.gamma0 * pr * rr;
+  T wl = M::sqrt(cl);
+  T wr = M::sqrt(cr);
+  T p_star = fmax2(M::div((wr * pl + wl * pr) + wl * wr * (ul - ur), wl + wr), T(0));
+
+  T conv = T(1);
+  for (int it = 0; it < h.niter && conv > h.conv_tol; ++it) {
+    const T wwl = M::sqrt(cl * (T(1) + M::div(h.gamma6 * (p_star - pl), pl)));
+    const T wwr = M::sqrt(cr * (T(1) + M::div(h.gamma6 * (p_star - pr), pr)));
+    const T zl = M::div(T(2) * wwl * wwl * wwl, wwl * wwl + cl);
+    const T zr = M::div(T(2) * wwr * wwr * wwr, wwr * wwr + cr);
+    const T usl = ul - M::div(p_star - pl, wwl);
+    const T usr = ur + M::div(p_star - pr, wwr);
+    const T delp = fmax2(M::div(zr * zl, zr + zl) * (usl - usr), -p_star);
+    p_star = p_star + delp;
+    conv = fabs_(M::div(delp, p_star + h.smallpp));
+  }
+
+  wl = M::sqrt(cl * (T(1) + M::div(h.gamma6 * (p_star - pl), pl)));
+  wr = M::sqrt(cr * (T(1) + M::div(h.gamma6 * (p_star - pr), pr)));
+  const T u_star = T(0.5) * (ul + M::div(pl - p_star, wl) + ur - M::div(pr - p_star, wr));
+
+  // upwind side of the contact (sgnm = +-1)
+  const bool right_going = u_star >= T(0);
+  const T ro = right_going ? rl : rr;
+  const T uo = right_going ? ul : ur;
+  const T po = right_going ? pl : pr;
+  const T wo = right_going ? wl : wr;
+  const T co = fmax2(h.smallc, M::sqrt(fabs_(M::div(h.gamma0 * po, ro))));
+  const T r_star = fmax2(M::div(ro, T(1) + M::div(ro * (po - p_star), wo * wo)), h.smallr);
+  const T c_star = fmax2(h.smallc, M::sqrt(fabs_(M::div(h.gamma0 * p_star, r_star))));
+
+  const T s_uo = signed_by(right_going, uo);
+  T spout = co - s_uo;
+  T spin = c_star - signed_by(right_going, u_star);
+  const T ushock = M::div(wo, ro) - s_uo;
+  if (p_star >= po) {
+    spin = ushock;
+    spout = ushock;
+  }
+  const T scr = fmax2(spout - spin, h.smallc + fabs_(spout + spin));
+  const T frac = T(0.5) * (T(1) + M::div(spout + spin, scr));
+
+  T gr = frac * r_star + (T(1) - frac) * ro;
+  T gu = frac * u_star + (T(1) - frac) * uo;
+  T gp = frac * p_star + (T(1) - frac) * po;
+  if (spout < T(0)) {
+    gr = ro; gu = uo; gp = po;
+  }
+  if (spin > T(0)) {
+    gr = r_star; gu = u_star; gp = p_star;
+  }
+  g[0] = gr;
+  g[1] = gp;
+  g[2] = gu;
+  g[3] = right_going ? ql[3] : qr[3];
+  godunov_flux(h, g, f);
+}
+
+// HLLC solver (riemann_2d_hllc, m_utils.f90:534-644).  The reference never
+// selects it in the step; here riemann_solver == 2 does.
+template <class T, class M>
+__device__ __forceinline__ void riemann_hllc(const Hydro<T>& h, const T ql[4], const T qr[4],
+                                             T g[4], T f[4]) {
+  const T H = T(0.5);
+  const T rl = fmax2(ql[0], h.smallr);
+  const T pl = fmax2(ql[1], rl * h.smallp);
+  const T ul = ql[2];
+  T ekl = H * rl * ul * ul;
+  ekl = ekl + H * rl * ql[3] * ql[3];
+  const T el = pl * h.entho + ekl;
+  const T rr = fmax2(qr[0], h.smallr);
+  const T pr = fmax2(qr[1], rr * h.smallp);
+  const T ur = qr[2];
+  T ekr = H * rr * ur * ur;
+  ekr = ekr + H * rr * qr[3] * qr[3];
+  const T er = pr * h.entho + ekr;
+
+  const T cfl_ = M::sqrt(fmax2(M::div(h.gamma0 * pl, rl), h.smallc * h.smallc));
+  const T cfr = M::sqrt(fmax2(M::div(h.gamma0 * pr, rr), h.smallc * h.smallc));
+  const T cmax = fmax2(cfl_, cfr);
+  const T SL = fmin2(ul, ur) - cmax;
+  const T SR = fmax2(ul, ur) + cmax;
+  const T rcl = rl * (ul - SL);
+  const T rcr = rr * (SR - ur);
+  const T ustar = M::div(rcr * ur + rcl * ul + (pl - pr), rcr + rcl);
+  const T pstar = M::div(rcr * pl + rcl * pr + rcl * rcr * (ul - ur), rcr + rcl);
+  const T rstl = M::div(rl * (SL - ul), SL - ustar);
+  const T estl = M::div((SL - ul) * el - pl * ul + pstar * ustar, SL - ustar);
+  const T rstr = M::div(rr * (SR - ur), SR - ustar);
+  const T estr = M::div((SR - ur) * er - pr * ur + pstar * ustar, SR - ustar);
+
+  T ro, uo, po, eo;
+  if (SL > T(0)) {
+    ro = rl; uo = ul; po = pl; eo = el;
+  } else if (ustar > T(0)) {
+    ro = rstl; uo = ustar; po = pstar; eo = estl;
+  } else if (SR > T(0)) {
+    ro = rstr; uo = ustar; po = pstar; eo = estr;
+  } else {
+    ro = rr; uo = ur; po = pr; eo = er;
+  }
+  g[0] = g[1] = g[2] = g[3] = T(0);
+  f[0] = ro * uo;
+  f[2] = ro * uo * uo + po;
+  f[1] = (eo + po) * uo;
+  f[3] = f[0] > T(0) ? f[0] * ql[3] : f[0] * qr[3];
+}
+
+// Solver selection at compile time (riemannSolverType, serial_ref/m_constants.f90:21-23).
+template <class T, class M, int SOLVER>
+__device__ __forceinline__ void riemann(const Hydro<T>& h, const T ql[4], const T qr[4], T g[4], T f[4]) {
+  if constexpr (SOLVER == 2)
+    riemann_hllc<T, M>(h, ql, qr, g, f);
+  else
+    riemann_two_shock<T, M>(h, ql, qr, g, f);
+}
+
+// Flux across a y-face: the x-solver with the velocity components exchanged
+// on input and output (serial_ref/m_run.f90:463-470).
+template <class T, class M, int SOLVER>
+__device__ __forceinline__ void riemann_y(const Hydro<T>& h, const T ql[4], const T qr[4], T f[4]) {
+  const T l[4] = {ql[0], ql[1], ql[3], ql[2]};
+  const T r[4] = {qr[0], qr[1], qr[3], qr[2]};
+  T g[4], fs[4];
+  riemann<T, M, SOLVER>(h, l, r, g, fs);
+  f[0] = fs[0];
+  f[1] = fs[1];
+  f[2] = fs[3];
+  f[3] = fs[2];
+}
+
+}  // namespace e2d

@@ -1,0 +1,191 @@
+/*
+ * euler2d_hip.h - C-ABI of the MI355X (gfx950) implementation of the euler2d
+ * hot path: the 2-D unsplit MUSCL-Hancock Godunov step (primitive conversion,
+ * slopes, trace, approximate two-shock Riemann solver, conservative update),
+ * the ghost-cell boundary fill and the CFL time-step max-reduction.
+ *
+ * The reference (pkestene/euler2d_cudaFortran) has no FFI: its host program
+ * calls Fortran module procedures that share module-global arrays.  Each entry
+ * point below replaces one of those procedures and cites it.  Every function
+ * returns 0 on success and a negative E2D_ERR_* code on failure; the message
+ * of the last failure on the calling thread is in e2d_last_error().  All
+ * arguments are validated before any HIP call is made.
+ *
+ * State layout (host side, what upload/download exchange): the reference's
+ * Fortran array u(isize, jsize, 4), column-major, isize = nx+4, jsize = ny+4
+ * (2 ghost layers each side, serial_ref/m_run.f90:105-109), variable planes
+ * in the order rho, E, rho*u, rho*v (serial_ref/m_constants.f90:11-14),
+ * always float64 on the interface whatever the device precision.
+ *
+ * Threading: a context is not thread-safe; it owns one HIP stream.  Calls are
+ * stream-ordered; only e2d_compute_dt, e2d_get_dt, e2d_download*, e2d_sums and
+ * e2d_synchronize wait for the device.
+ */
+#ifndef EULER2D_HIP_H
+#define EULER2D_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define E2D_VERSION 1
+
+/* status codes */
+#define E2D_OK 0
+#define E2D_ERR_ARG (-1)      /* invalid argument (null pointer, bad index, bad parameter) */
+#define E2D_ERR_HIP (-2)      /* a HIP runtime call failed */
+#define E2D_ERR_NOMEM (-3)    /* device or host allocation failed */
+#define E2D_ERR_STATE (-4)    /* call not valid in the context's current state */
+
+/* implementation_version: cuf/m_HydroConstants.cuf:51-53 */
+#define E2D_IMPL_FUSED 0        /* one fused kernel per step (LDS-free register marching) */
+#define E2D_IMPL_MATERIALISED 1 /* q, slopes, face states and fluxes stored in HBM */
+
+/* precision */
+#define E2D_PREC_FP64 0   /* FP64 state and arithmetic (serial_ref built with fp_kind = dp_kind) */
+#define E2D_PREC_FP32 1   /* FP32 state and arithmetic (serial_ref default kind) */
+#define E2D_PREC_MIXED 2  /* FP32 state and flux arithmetic, FP64 update/dt/sums */
+
+/* riemann_solver: serial_ref/m_constants.f90:21-23 */
+#define E2D_RIEMANN_APPROX 0
+#define E2D_RIEMANN_HLLC 2
+
+/* initial conditions for e2d_init_problem */
+#define E2D_PROBLEM_IMPLODE 0 /* init_implode, serial_ref/m_run.f90:524-555 */
+#define E2D_PROBLEM_DIAG 1    /* x+y<1: rho=1,p=1 else rho=0.125,p=0.1; u=v=0 */
+#define E2D_PROBLEM_BUMPS 2   /* DIAG plus 16 seeded Gaussian bumps in rho and p */
+
+/* stages kept by the materialised implementation, for e2d_download_stage */
+#define E2D_STAGE_Q 0
+#define E2D_STAGE_FX 1
+#define E2D_STAGE_FY 2
+#define E2D_STAGE_DQX 3
+#define E2D_STAGE_DQY 4
+#define E2D_STAGE_QXM 5 /* XMAX face state (left state at the cell's right interface) */
+#define E2D_STAGE_QXP 6 /* XMIN face state */
+#define E2D_STAGE_QYM 7 /* YMAX face state */
+#define E2D_STAGE_QYP 8 /* YMIN face state */
+
+/* pointwise device numerics, for e2d_pointwise (input/output widths per item) */
+#define E2D_FN_PRIM 1    /* u[4]                      -> q[4], c            */
+#define E2D_FN_SLOPE 2   /* q, q+x, q-x, q+y, q-y [20] -> dqX[4], dqY[4]     */
+#define E2D_FN_TRACE 3   /* q, dqX, dqY, dtdx, dtdy, face [15] -> qface[4]   */
+#define E2D_FN_RIEMANN 4 /* qleft[4], qright[4]       -> qgdnv[4], flux[4]  */
+#define E2D_FN_HLLC 5    /* qleft[4], qright[4]       -> qgdnv[4], flux[4]  */
+#define E2D_FN_CMPFLX 6  /* qgdnv[4]                  -> flux[4]            */
+#define E2D_FN_EOS 7     /* rho, eint                 -> p, c               */
+
+/*
+ * Run parameters: the namelist groups &RUN/&MESH/&HYDRO/&OTHER of the
+ * reference (serial_ref/m_run.f90:71-74, cuf/m_HydroParameters.cuf:86-90)
+ * minus the loop controls, which stay with the host.  Derived parameters
+ * (dx, dy, smallp, smallpp, gamma6; serial_ref/m_run.f90:111-124) are
+ * computed inside, in the run's precision.
+ */
+typedef struct e2d_params {
+  int32_t nx, ny;
+  int32_t boundary_type_xmin, boundary_type_xmax; /* 1 reflective, 2 absorbing, other periodic */
+  int32_t boundary_type_ymin, boundary_type_ymax;
+  int32_t slope_type;    /* 0 first order, 1 minmod, 2 monotonized central */
+  int32_t niter_riemann; /* Newton iterations cap of the two-shock solver */
+  int32_t iorder;        /* read and unused, as in the reference */
+  int32_t implementation_version;
+  int32_t precision;
+  int32_t riemann_solver;
+  double gamma0, cfl, smallr, smallc;
+} e2d_params;
+
+typedef struct e2d_ctx e2d_ctx;
+
+/* Library version (E2D_VERSION) and whether a HIP device is usable (1/0). */
+int e2d_version(void);
+int e2d_device_available(void);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *e2d_last_error(void);
+
+/* Replaces initHydroParameters_gpu + the allocations of initHydroRun
+ * (cuf/m_HydroParameters_gpu.cuf:67-103, cuf/m_HydroRun_gpu.cuf:83-120):
+ * validates p, allocates the two state buffers (u = buffer 0, u2 = buffer 1)
+ * and, for the materialised version, the stage arrays.  Buffer 1 starts at
+ * zero, as the reference's never-initialised u2 (serial_ref/m_run.f90:155). */
+int e2d_create(const e2d_params *p, e2d_ctx **out);
+
+/* Replaces cleanupHydroRun (cuf/m_HydroRun_gpu.cuf:123-140). */
+int e2d_destroy(e2d_ctx *ctx);
+
+/* Replaces the initial condition + upload of initHydroRun
+ * (serial_ref/m_run.f90:160-167, cuf/m_HydroRun_gpu.cuf:105-118), computed on
+ * the device in the run's precision; zeroes buffer 1, resets the running
+ * invDt and the step counter.  seed is used by E2D_PROBLEM_BUMPS only. */
+int e2d_init_problem(e2d_ctx *ctx, int problem, uint64_t seed);
+
+/* d_u = h_u (cuf/m_HydroRun_gpu.cuf:117) and h_u = d_u / d_u2
+ * (cuf/main_gpu.cuf:58-62) for buffer 0 (u) or 1 (u2). */
+int e2d_upload(e2d_ctx *ctx, int buf, const double *h_u);
+int e2d_download(e2d_ctx *ctx, int buf, double *h_u);
+
+/* Replaces compute_dt (serial_ref/m_run.f90:248-312, cuf/m_HydroRun_gpu.cuf:198-300):
+ * useU selects buffer 0 (useU == 0) or 1, the reduction window is
+ * [3, size-3] (1-based), the running maximum of invDt over all calls is kept
+ * in the context (the SAVE'd local of the reference) and dt = cfl / invDt is
+ * returned.  Synchronous. */
+int e2d_compute_dt(e2d_ctx *ctx, int useU, double *dt);
+
+/* Replaces make_boundaries_gpu (cuf/m_HydroRun_gpu.cuf:1041-1132) /
+ * make_boundaries (serial_ref/m_run.f90:561-648): fills the non-corner ghost
+ * cells of buffer buf; corners are never written. */
+int e2d_make_boundaries(e2d_ctx *ctx, int buf);
+
+/* Replaces godunov_unsplit (serial_ref/m_run.f90:187-243,
+ * cuf/m_HydroRun_gpu.cuf:306-452): one step with the host's dt, buffer 0 -> 1
+ * for even nStep, 1 -> 0 for odd; boundaries of the input buffer are filled
+ * first; nothing is copied from the input to the output buffer. */
+int e2d_godunov_unsplit(e2d_ctx *ctx, int nStep, double dt);
+
+/* The main-loop body of the reference (serial_ref/main.f90:66-74,
+ * cuf/main_gpu.cuf:67-73) nsteps times without host round trips: per step
+ * dt = cfl / invDt from the device-resident running maximum, the Godunov
+ * step, and invDt updated from the new state.  *nStep is the step counter of
+ * the host loop (selects the ping-pong direction) and is advanced by nsteps.
+ * Equivalent, bit for bit, to calling e2d_compute_dt + e2d_godunov_unsplit
+ * nsteps times. */
+int e2d_run_steps(e2d_ctx *ctx, int nsteps, int *nStep);
+
+/* dt that the next step will use (cfl / running invDt); synchronous. */
+int e2d_get_dt(e2d_ctx *ctx, double *dt);
+
+/* Running invDt (the only state the reference loop carries besides u, u2):
+ * read/set for checkpoint and restart. */
+int e2d_get_invdt(e2d_ctx *ctx, double *invdt);
+int e2d_set_invdt(e2d_ctx *ctx, double invdt);
+
+/* FP64 sums over the interior of buffer buf of (rho, E, rho*u, rho*v),
+ * in a fixed order (deterministic). */
+int e2d_sums(e2d_ctx *ctx, int buf, double out[4]);
+
+/* Materialised version only: download a stage array of the last step
+ * (layout as the state; q, dq and face arrays (isize,jsize,4), fx/fy as the
+ * reference's fx/fy arrays, serial_ref/m_run.f90:17). */
+int e2d_download_stage(e2d_ctx *ctx, int stage, double *h);
+
+/* Wait for all work of the context. */
+int e2d_synchronize(e2d_ctx *ctx);
+
+/* Device pointer and row pitch (in elements) of a state buffer; element
+ * (i, j, v) (0-based) sits at ptr[(v * jsize + j) * pitch + i], in the
+ * precision's storage type (double for FP64, float for FP32/MIXED). */
+int e2d_device_buffer(e2d_ctx *ctx, int buf, void **dptr, int64_t *pitch);
+
+/* Pointwise device numerics over a batch, for known-answer tests of the
+ * device routines against the reference's (serial_ref/m_utils.f90).
+ * x holds n items of the input width of func, y receives n items of its
+ * output width (see E2D_FN_*); float64 host arrays; synchronous. */
+int e2d_pointwise(const e2d_params *p, int func, int n, const double *x, double *y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EULER2D_HIP_H */

@@ -1,0 +1,36 @@
+/*
+ * Test infrastructure only: the oracle (see cpu_ref.h).  Instantiates
+ * cpu_ref_impl.h for the reference's two kinds.
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "cpu_ref.h"
+
+/* variable and face indices: serial_ref/m_constants.f90:11-14, 51-54 */
+enum { ID = 1, IP = 2, IU = 3, IV = 4 };
+enum { FACE_XMIN = 0, FACE_XMAX = 1, FACE_YMIN = 2, FACE_YMAX = 3 };
+
+#define R double
+#define S f64
+#define SQRT sqrt
+#define FABS fabs
+#define LIT_F32(x) ((double)(x##f))
+#define LIT_F64(x) (x)
+#include "cpu_ref_impl.h"
+#undef R
+#undef S
+#undef SQRT
+#undef FABS
+#undef LIT_F32
+#undef LIT_F64
+
+#define R float
+#define S f32
+#define SQRT sqrtf
+#define FABS fabsf
+#define LIT_F32(x) (x##f)
+#define LIT_F64(x) (x##f)
+#include "cpu_ref_impl.h"

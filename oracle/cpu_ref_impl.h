@@ -1,0 +1,649 @@
+/*
+ * Test infrastructure only: body of the oracle, instantiated once per precision
+ * by cpu_ref.c with
+ *   R         the reference's fp_kind (double or float)
+ *   S         the symbol suffix (f64 or f32)
+ *   SQRT/FABS the libm routines of that precision
+ *   LIT_F64   an fp_kind literal (used by the benchmark initial condition)
+ *   LIT_F32   how a default-real (FP32) literal of the Fortran source enters
+ *             R arithmetic: (R)(x##f).  In the FP64 build `1e-6` and `0.14`
+ *             are FP32 constants widened to double (serial_ref/m_utils.f90:396,
+ *             serial_ref/m_run.f90:548); every other literal of the path is
+ *             exact in both precisions.
+ * Each function cites the reference lines it restates.  Fortran max/min are
+ * restated as (a > b ? a : b) / (a < b ? a : b), which is what the reference
+ * compiler emits (checked on signed zeros).
+ */
+
+#define CAT_(a, b) a##_##b
+#define CAT(a, b) CAT_(a, b)
+#define FN(name) CAT(name, S)
+
+typedef struct {
+  int nx, ny, isize, jsize;
+  int bc[4]; /* xmin, xmax, ymin, ymax */
+  int slope_type, niter_riemann, riemann_solver;
+  R gamma0, gamma6, cfl, smallr, smallc, smallp, smallpp, dx, dy;
+  R *u, *u2, *q, *fx, *fy;
+  R invDt; /* SAVEd running max of compute_dt (serial_ref/m_run.f90:257) */
+} FN(run);
+
+#define NBVAR 4
+/* 1-based Fortran indexing, as in the reference: a(i,j,v) */
+#define AT(a, i, j, v) (a)[((size_t)((v)-1) * (size_t)h->jsize + (size_t)((j)-1)) * (size_t)h->isize + (size_t)((i)-1)]
+
+static inline R FN(fmax2)(R a, R b) { return a > b ? a : b; }
+static inline R FN(fmin2)(R a, R b) { return a < b ? a : b; }
+#define MAX2 FN(fmax2)
+#define MIN2 FN(fmin2)
+#define ABS(x) FABS(x) /* Fortran abs clears the sign bit */
+
+/* eos: serial_ref/m_utils.f90:15-29 */
+static inline void FN(eos)(const FN(run) * h, R rho, R eint, R *p, R *c) {
+  *p = MAX2((h->gamma0 - (R)1.0) * rho * eint, rho * h->smallp);
+  *c = SQRT(h->gamma0 * *p / rho);
+}
+
+/* computePrimitives: serial_ref/m_utils.f90:35-64 (order ID, IP, IU, IV) */
+static inline void FN(prim)(const FN(run) * h, const R u[4], R q[4], R *c) {
+  q[0] = MAX2(u[0], h->smallr);
+  q[2] = u[2] / q[0];
+  q[3] = u[3] / q[0];
+  R eken = (R)0.5 * (q[2] * q[2] + q[3] * q[3]);
+  R e = u[1] / q[0] - eken;
+  FN(eos)(h, q[0], e, &q[1], c);
+}
+
+/* slope_unsplit_hydro_2d: serial_ref/m_utils.f90:268-342.
+ * dq[0..3] = X slopes, dq[4..7] = Y slopes. */
+static inline void FN(slope)(const FN(run) * h, const R q[4], const R qpx[4], const R qmx[4],
+                             const R qpy[4], const R qmy[4], R dq[8]) {
+  if (h->slope_type == 1 || h->slope_type == 2) {
+    const R st = (R)h->slope_type;
+    for (int v = 0; v < NBVAR; ++v) {
+      R dlft = st * (q[v] - qmx[v]);
+      R drgt = st * (qpx[v] - q[v]);
+      R dcen = (R)0.5 * (qpx[v] - qmx[v]);
+      R dsgn = (dcen >= (R)0) ? (R)1 : (R)-1;
+      R slop = MIN2(ABS(dlft), ABS(drgt));
+      R dlim = slop;
+      if ((dlft * drgt) <= (R)0) dlim = (R)0;
+      dq[v] = dsgn * MIN2(dlim, ABS(dcen));
+
+      dlft = st * (q[v] - qmy[v]);
+      drgt = st * (qpy[v] - q[v]);
+      dcen = (R)0.5 * (qpy[v] - qmy[v]);
+      dsgn = (dcen >= (R)0) ? (R)1 : (R)-1;
+      slop = MIN2(ABS(dlft), ABS(drgt));
+      dlim = slop;
+      if ((dlft * drgt) <= (R)0) dlim = (R)0;
+      dq[4 + v] = dsgn * MIN2(dlim, ABS(dcen));
+    }
+  } else {
+    for (int v = 0; v < 8; ++v) dq[v] = (R)0;
+  }
+}
+
+/* trace_unsplit_2d_along_dir: serial_ref/m_utils.f90:177-254 */
+static inline void FN(trace)(const FN(run) * h, const R q[4], const R dqX[4], const R dqY[4], R dtdx,
+                             R dtdy, int faceId, R qf[4]) {
+  const R r = q[0], p = q[1], u = q[2], v = q[3];
+  const R drx = dqX[0], dpx = dqX[1], dux = dqX[2], dvx = dqX[3];
+  const R dry = dqY[0], dpy = dqY[1], duy = dqY[2], dvy = dqY[3];
+  const R sr0 = -u * drx - v * dry - (dux + dvy) * r;
+  const R sp0 = -u * dpx - v * dpy - (dux + dvy) * h->gamma0 * p;
+  const R su0 = -u * dux - v * duy - (dpx) / r;
+  const R sv0 = -u * dvx - v * dvy - (dpy) / r;
+  const R H = (R)0.5;
+  if (faceId == FACE_XMIN) {
+    qf[0] = r - H * drx + sr0 * dtdx * H;
+    qf[1] = p - H * dpx + sp0 * dtdx * H;
+    qf[2] = u - H * dux + su0 * dtdx * H;
+    qf[3] = v - H * dvx + sv0 * dtdx * H;
+    qf[0] = MAX2(h->smallr, qf[0]);
+  }
+  if (faceId == FACE_XMAX) {
+    qf[0] = r + H * drx + sr0 * dtdx * H;
+    qf[1] = p + H * dpx + sp0 * dtdx * H;
+    qf[2] = u + H * dux + su0 * dtdx * H;
+    qf[3] = v + H * dvx + sv0 * dtdx * H;
+    qf[0] = MAX2(h->smallr, qf[0]);
+  }
+  if (faceId == FACE_YMIN) {
+    qf[0] = r - H * dry + sr0 * dtdy * H;
+    qf[1] = p - H * dpy + sp0 * dtdy * H;
+    qf[2] = u - H * duy + su0 * dtdy * H;
+    qf[3] = v - H * dvy + sv0 * dtdy * H;
+    qf[0] = MAX2(h->smallr, qf[0]);
+  }
+  if (faceId == FACE_YMAX) {
+    qf[0] = r + H * dry + sr0 * dtdy * H;
+    qf[1] = p + H * dpy + sp0 * dtdy * H;
+    qf[2] = u + H * duy + su0 * dtdy * H;
+    qf[3] = v + H * dvy + sv0 * dtdy * H;
+    qf[0] = MAX2(h->smallr, qf[0]);
+  }
+}
+
+/* cmpflx: serial_ref/m_utils.f90:498-526 */
+static inline void FN(cmpflx)(const FN(run) * h, const R g[4], R f[4]) {
+  f[0] = g[0] * g[2];
+  f[2] = f[0] * g[2] + g[1];
+  f[3] = f[0] * g[3];
+  R entho = (R)1.0 / (h->gamma0 - (R)1.0);
+  R ekin = (R)0.5 * g[0] * (g[2] * g[2] + g[3] * g[3]);
+  R etot = g[1] * entho + ekin;
+  f[1] = g[2] * (etot + g[1]);
+}
+
+/* riemann_2d: serial_ref/m_utils.f90:349-492.  Returns the Newton iteration
+ * count (not part of the reference interface; used to grade test coverage). */
+static inline int FN(riemann)(const FN(run) * h, const R ql[4], const R qr[4], R g[4], R f[4]) {
+  const R rl = MAX2(ql[0], h->smallr);
+  const R ul = ql[2];
+  const R pl = MAX2(ql[1], rl * h->smallp);
+  const R rr = MAX2(qr[0], h->smallr);
+  const R ur = qr[2];
+  const R pr = MAX2(qr[1], rr * h->smallp);
+
+  const R cl = h->gamma0 * pl * rl;
+  const R cr = h->gamma0 * pr * rr;
+
+  R wl = SQRT(cl);
+  R wr = SQRT(cr);
+  R pstar = MAX2(((wr * pl + wl * pr) + wl * wr * (ul - ur)) / (wl + wr), (R)0);
+  R pold = pstar;
+  R conv = (R)1;
+
+  int iter = 0;
+  while (iter < h->niter_riemann && conv > LIT_F32(1e-6)) {
+    R wwl = SQRT(cl * ((R)1 + h->gamma6 * (pold - pl) / pl));
+    R wwr = SQRT(cr * ((R)1 + h->gamma6 * (pold - pr) / pr));
+    R qlw = (R)2.0 * wwl * wwl * wwl / (wwl * wwl + cl);
+    R qrw = (R)2.0 * wwr * wwr * wwr / (wwr * wwr + cr);
+    R usl = ul - (pold - pl) / wwl;
+    R usr = ur + (pold - pr) / wwr;
+    R delp = MAX2(qrw * qlw / (qrw + qlw) * (usl - usr), -pold);
+    pold = pold + delp;
+    conv = ABS(delp / (pold + h->smallpp));
+    iter = iter + 1;
+  }
+
+  pstar = pold;
+  wl = SQRT(cl * ((R)1 + h->gamma6 * (pstar - pl) / pl));
+  wr = SQRT(cr * ((R)1 + h->gamma6 * (pstar - pr) / pr));
+
+  const R ustar = (R)0.5 * (ul + (pl - pstar) / wl + ur - (pr - pstar) / wr);
+  const R sgnm = (ustar >= (R)0) ? (R)1.0 : (R)-1.0;
+
+  R ro, uo, po, wo;
+  if (sgnm > (R)0) {
+    ro = rl; uo = ul; po = pl; wo = wl;
+  } else {
+    ro = rr; uo = ur; po = pr; wo = wr;
+  }
+  const R co = MAX2(h->smallc, SQRT(ABS(h->gamma0 * po / ro)));
+  const R rstar = MAX2(ro / ((R)1 + ro * (po - pstar) / (wo * wo)), h->smallr);
+  const R cstar = MAX2(h->smallc, SQRT(ABS(h->gamma0 * pstar / rstar)));
+
+  R spout = co - sgnm * uo;
+  R spin = cstar - sgnm * ustar;
+  const R ushock = wo / ro - sgnm * uo;
+  if (pstar >= po) {
+    spin = ushock;
+    spout = ushock;
+  }
+  const R scr = MAX2(spout - spin, h->smallc + ABS(spout + spin));
+  const R frac = (R)0.5 * ((R)1 + (spout + spin) / scr);
+
+  g[0] = frac * rstar + ((R)1 - frac) * ro;
+  g[2] = frac * ustar + ((R)1 - frac) * uo;
+  g[1] = frac * pstar + ((R)1 - frac) * po;
+  if (spout < (R)0) {
+    g[0] = ro; g[2] = uo; g[1] = po;
+  }
+  if (spin > (R)0) {
+    g[0] = rstar; g[2] = ustar; g[1] = pstar;
+  }
+  g[3] = (sgnm > (R)0) ? ql[3] : qr[3];
+
+  FN(cmpflx)(h, g, f);
+  return iter;
+}
+
+/* riemann_2d_hllc: serial_ref/m_utils.f90:534-644 */
+static inline void FN(hllc)(const FN(run) * h, const R ql[4], const R qr[4], R g[4], R f[4]) {
+  const R H = (R)0.5;
+  const R entho = (R)1 / (h->gamma0 - (R)1);
+  const R rl = MAX2(ql[0], h->smallr);
+  const R pl = MAX2(ql[1], rl * h->smallp);
+  const R ul = ql[2];
+  R ecinl = H * rl * ul * ul;
+  ecinl = ecinl + H * rl * ql[3] * ql[3];
+  const R etotl = pl * entho + ecinl;
+  const R ptotl = pl;
+
+  const R rr = MAX2(qr[0], h->smallr);
+  const R pr = MAX2(qr[1], rr * h->smallp);
+  const R ur = qr[2];
+  R ecinr = H * rr * ur * ur;
+  ecinr = ecinr + H * rr * qr[3] * qr[3];
+  const R etotr = pr * entho + ecinr;
+  const R ptotr = pr;
+
+  const R cfastl = SQRT(MAX2(h->gamma0 * pl / rl, h->smallc * h->smallc));
+  const R cfastr = SQRT(MAX2(h->gamma0 * pr / rr, h->smallc * h->smallc));
+
+  const R SL = MIN2(ul, ur) - MAX2(cfastl, cfastr);
+  const R SR = MAX2(ul, ur) + MAX2(cfastl, cfastr);
+
+  const R rcl = rl * (ul - SL);
+  const R rcr = rr * (SR - ur);
+  const R rcsum = MAX2(rcl + rcr, h->smallr);
+  (void)rcsum; /* computed and unused in the reference as well (:594) */
+
+  const R ustar = (rcr * ur + rcl * ul + (ptotl - ptotr)) / (rcr + rcl);
+  const R ptotstar = (rcr * ptotl + rcl * ptotr + rcl * rcr * (ul - ur)) / (rcr + rcl);
+
+  const R rstarl = rl * (SL - ul) / (SL - ustar);
+  const R etotstarl = ((SL - ul) * etotl - ptotl * ul + ptotstar * ustar) / (SL - ustar);
+  const R rstarr = rr * (SR - ur) / (SR - ustar);
+  const R etotstarr = ((SR - ur) * etotr - ptotr * ur + ptotstar * ustar) / (SR - ustar);
+
+  R ro, uo, ptoto, etoto;
+  if (SL > (R)0) {
+    ro = rl; uo = ul; ptoto = ptotl; etoto = etotl;
+  } else if (ustar > (R)0) {
+    ro = rstarl; uo = ustar; ptoto = ptotstar; etoto = etotstarl;
+  } else if (SR > (R)0) {
+    ro = rstarr; uo = ustar; ptoto = ptotstar; etoto = etotstarr;
+  } else {
+    ro = rr; uo = ur; ptoto = ptotr; etoto = etotr;
+  }
+  g[0] = g[1] = g[2] = g[3] = (R)0;
+  f[0] = ro * uo;
+  f[2] = ro * uo * uo + ptoto;
+  f[1] = (etoto + ptoto) * uo;
+  f[3] = (f[0] > (R)0) ? f[0] * ql[3] : f[0] * qr[3];
+}
+
+static inline void FN(solve)(const FN(run) * h, const R ql[4], const R qr[4], R g[4], R f[4]) {
+  if (h->riemann_solver == 2)
+    FN(hllc)(h, ql, qr, g, f);
+  else
+    (void)FN(riemann)(h, ql, qr, g, f);
+}
+
+/* ------------------------------------------------------------------------ */
+
+void *FN(oracle_create)(int nx, int ny, const int bc[4], int slope_type, int niter_riemann,
+                        int riemann_solver, double gamma0, double cfl, double smallr, double smallc) {
+  FN(run) *h = (FN(run) *)calloc(1, sizeof(FN(run)));
+  if (!h) return NULL;
+  const int ghostWidth = 2;
+  h->nx = nx;
+  h->ny = ny;
+  /* serial_ref/m_run.f90:103-124 */
+  h->isize = nx + 2 * ghostWidth;
+  h->jsize = ny + 2 * ghostWidth;
+  for (int k = 0; k < 4; ++k) h->bc[k] = bc[k];
+  h->slope_type = slope_type;
+  h->niter_riemann = niter_riemann;
+  h->riemann_solver = riemann_solver;
+  h->gamma0 = (R)gamma0;
+  h->cfl = (R)cfl;
+  h->smallr = (R)smallr;
+  h->smallc = (R)smallc;
+  h->dx = ((R)1.0 - (R)0.0) / (R)nx;
+  h->dy = ((R)1.0 - (R)0.0) / (R)ny;
+  h->smallp = h->smallc * h->smallc / h->gamma0;
+  h->smallpp = h->smallr * h->smallp;
+  h->gamma6 = (h->gamma0 + (R)1) / ((R)2 * h->gamma0);
+  size_t n = (size_t)h->isize * (size_t)h->jsize * NBVAR;
+  h->u = (R *)calloc(n, sizeof(R));
+  h->u2 = (R *)calloc(n, sizeof(R)); /* zero, as the reference's unwritten u2 */
+  h->q = (R *)calloc(n, sizeof(R));
+  h->fx = (R *)calloc(n, sizeof(R));
+  h->fy = (R *)calloc(n, sizeof(R));
+  h->invDt = (R)0;
+  if (!h->u || !h->u2 || !h->q || !h->fx || !h->fy) {
+    FN(oracle_destroy)(h);
+    return NULL;
+  }
+  return h;
+}
+
+void FN(oracle_destroy)(void *hv) {
+  FN(run) *h = (FN(run) *)hv;
+  if (!h) return;
+  free(h->u);
+  free(h->u2);
+  free(h->q);
+  free(h->fx);
+  free(h->fy);
+  free(h);
+}
+
+/* init_implode: serial_ref/m_run.f90:524-555 (tmp evaluated in default real);
+ * diag: the benchmark initial condition (x+y<1 high), same cell indexing. */
+void FN(oracle_init)(void *hv, int problem) {
+  FN(run) *h = (FN(run) *)hv;
+  const int ghostWidth = 2;
+  for (int j = 1; j <= h->jsize; ++j)
+    for (int i = 1; i <= h->isize; ++i) {
+      if (problem == ORACLE_IC_DIAG) {
+        R x = (R)(i - ghostWidth - 1) / (R)h->nx;
+        R y = (R)(j - ghostWidth - 1) / (R)h->ny;
+        AT(h->u, i, j, IU) = (R)0;
+        AT(h->u, i, j, IV) = (R)0;
+        if (x + y < (R)1) {
+          AT(h->u, i, j, ID) = (R)1;
+          AT(h->u, i, j, IP) = (R)1 / (h->gamma0 - (R)1);
+        } else {
+          AT(h->u, i, j, ID) = (R)0.125;
+          AT(h->u, i, j, IP) = LIT_F64(0.1) / (h->gamma0 - (R)1);
+        }
+      } else {
+        float tmp = 1.0f * (float)(i - ghostWidth - 1) / (float)h->nx +
+                    1.0f * (float)(j - ghostWidth - 1) / (float)h->ny;
+        if ((R)tmp > (R)0.5) {
+          AT(h->u, i, j, ID) = (R)1.0;
+          AT(h->u, i, j, IP) = (R)1.0 / (h->gamma0 - (R)1.0);
+          AT(h->u, i, j, IU) = (R)0.0;
+          AT(h->u, i, j, IV) = (R)0.0;
+        } else {
+          AT(h->u, i, j, ID) = (R)0.125;
+          AT(h->u, i, j, IP) = LIT_F32(0.14) / (h->gamma0 - (R)1.0);
+          AT(h->u, i, j, IU) = (R)0.0;
+          AT(h->u, i, j, IV) = (R)0.0;
+        }
+      }
+    }
+  size_t n = (size_t)h->isize * (size_t)h->jsize * NBVAR;
+  memset(h->u2, 0, n * sizeof(R));
+  h->invDt = (R)0;
+}
+
+/* compute_dt: serial_ref/m_run.f90:248-312.  Window [3, size-3], running max. */
+double FN(oracle_compute_dt)(void *hv, int useU) {
+  FN(run) *h = (FN(run) *)hv;
+  const int ghostWidth = 2;
+  R *a = (useU == 0) ? h->u : h->u2;
+  for (int j = ghostWidth + 1; j <= h->jsize - ghostWidth - 1; ++j)
+    for (int i = ghostWidth + 1; i <= h->isize - ghostWidth - 1; ++i) {
+      R uLoc[4] = {AT(a, i, j, ID), AT(a, i, j, IP), AT(a, i, j, IU), AT(a, i, j, IV)};
+      R qLoc[4], c;
+      FN(prim)(h, uLoc, qLoc, &c);
+      R vx = c + ABS(qLoc[2]);
+      R vy = c + ABS(qLoc[3]);
+      h->invDt = MAX2(h->invDt, vx / h->dx + vy / h->dy);
+    }
+  return (double)(h->cfl / h->invDt);
+}
+
+double FN(oracle_get_invdt)(void *hv) { return (double)((FN(run) *)hv)->invDt; }
+void FN(oracle_set_invdt)(void *hv, double v) { ((FN(run) *)hv)->invDt = (R)v; }
+
+/* make_boundaries: serial_ref/m_run.f90:561-648.  Corners are never written. */
+void FN(oracle_make_boundaries)(void *hv, int which) {
+  FN(run) *h = (FN(run) *)hv;
+  const int gw = 2, nx = h->nx, ny = h->ny;
+  const int imin = 1, jmin = 1, imax = nx + 2 * gw, jmax = ny + 2 * gw;
+  R *d = (which == 0) ? h->u : h->u2;
+  for (int j = jmin + gw; j <= jmax - gw; ++j)
+    for (int v = 1; v <= NBVAR; ++v)
+      for (int i = 1; i <= gw; ++i) {
+        R dir = (R)1;
+        int i0;
+        if (h->bc[0] == 1) {
+          i0 = 2 * gw + 1 - i;
+          if (v == IU) dir = (R)-1;
+        } else if (h->bc[0] == 2) {
+          i0 = gw + 1;
+        } else {
+          i0 = nx + i;
+        }
+        AT(d, i, j, v) = AT(d, i0, j, v) * dir;
+      }
+  for (int j = jmin + gw; j <= jmax - gw; ++j)
+    for (int v = 1; v <= NBVAR; ++v)
+      for (int i = nx + gw + 1; i <= nx + 2 * gw; ++i) {
+        R dir = (R)1;
+        int i0;
+        if (h->bc[1] == 1) {
+          i0 = 2 * nx + 2 * gw + 1 - i;
+          if (v == IU) dir = (R)-1;
+        } else if (h->bc[1] == 2) {
+          i0 = nx + gw;
+        } else {
+          i0 = i - nx;
+        }
+        AT(d, i, j, v) = AT(d, i0, j, v) * dir;
+      }
+  for (int i = imin + gw; i <= imax - gw; ++i)
+    for (int v = 1; v <= NBVAR; ++v)
+      for (int j = 1; j <= gw; ++j) {
+        R dir = (R)1;
+        int j0;
+        if (h->bc[2] == 1) {
+          j0 = 2 * gw + 1 - j;
+          if (v == IV) dir = (R)-1;
+        } else if (h->bc[2] == 2) {
+          j0 = gw + 1;
+        } else {
+          j0 = ny + j;
+        }
+        AT(d, i, j, v) = AT(d, i, j0, v) * dir;
+      }
+  for (int i = imin + gw; i <= imax - gw; ++i)
+    for (int v = 1; v <= NBVAR; ++v)
+      for (int j = ny + gw + 1; j <= ny + 2 * gw; ++j) {
+        R dir = (R)1;
+        int j0;
+        if (h->bc[3] == 1) {
+          j0 = 2 * ny + 2 * gw + 1 - j;
+          if (v == IV) dir = (R)-1;
+        } else if (h->bc[3] == 2) {
+          j0 = ny + gw;
+        } else {
+          j0 = j - ny;
+        }
+        AT(d, i, j, v) = AT(d, i, j0, v) * dir;
+      }
+}
+
+/* convertToPrimitives: serial_ref/m_run.f90:317-358 (all cells) */
+static void FN(convert_to_primitives)(FN(run) * h, const R *ud) {
+  for (int j = 1; j <= h->jsize; ++j)
+    for (int i = 1; i <= h->isize; ++i) {
+      R uLoc[4] = {AT(ud, i, j, ID), AT(ud, i, j, IP), AT(ud, i, j, IU), AT(ud, i, j, IV)};
+      R qLoc[4], c;
+      FN(prim)(h, uLoc, qLoc, &c);
+      AT(h->q, i, j, ID) = qLoc[0];
+      AT(h->q, i, j, IP) = qLoc[1];
+      AT(h->q, i, j, IU) = qLoc[2];
+      AT(h->q, i, j, IV) = qLoc[3];
+    }
+}
+
+static inline void FN(load_q)(const FN(run) * h, int i, int j, R out[4]) {
+  out[0] = AT(h->q, i, j, ID);
+  out[1] = AT(h->q, i, j, IP);
+  out[2] = AT(h->q, i, j, IU);
+  out[3] = AT(h->q, i, j, IV);
+}
+
+/* computeAndStoreFluxes: serial_ref/m_run.f90:363-484 */
+static void FN(compute_and_store_fluxes)(FN(run) * h, R dt) {
+  const int gw = 2;
+  const R dtdx = dt / h->dx;
+  const R dtdy = dt / h->dy;
+  for (int j = gw + 1; j <= h->jsize - gw + 1; ++j)
+    for (int i = gw + 1; i <= h->isize - gw + 1; ++i) {
+      R qLoc[4], qLocN[4], qN0[4], qN1[4], qN2[4], qN3[4], dq[8], dqN[8];
+      R qleft[4], qright[4], qgdnv[4], flux_x[4], flux_y[4], tmp;
+
+      FN(load_q)(h, i, j, qLoc);
+      FN(load_q)(h, i + 1, j, qN0);
+      FN(load_q)(h, i - 1, j, qN1);
+      FN(load_q)(h, i, j + 1, qN2);
+      FN(load_q)(h, i, j - 1, qN3);
+      FN(slope)(h, qLoc, qN0, qN1, qN2, qN3, dq);
+
+      FN(load_q)(h, i - 1, j, qLocN);
+      FN(load_q)(h, i, j, qN0);
+      FN(load_q)(h, i - 2, j, qN1);
+      FN(load_q)(h, i - 1, j + 1, qN2);
+      FN(load_q)(h, i - 1, j - 1, qN3);
+      FN(slope)(h, qLocN, qN0, qN1, qN2, qN3, dqN);
+
+      FN(trace)(h, qLoc, dq, dq + 4, dtdx, dtdy, FACE_XMIN, qright);
+      FN(trace)(h, qLocN, dqN, dqN + 4, dtdx, dtdy, FACE_XMAX, qleft);
+      FN(solve)(h, qleft, qright, qgdnv, flux_x);
+      for (int v = 0; v < NBVAR; ++v) AT(h->fx, i, j, v + 1) = flux_x[v] * dtdx;
+
+      FN(load_q)(h, i, j - 1, qLocN);
+      FN(load_q)(h, i + 1, j - 1, qN0);
+      FN(load_q)(h, i - 1, j - 1, qN1);
+      FN(load_q)(h, i, j, qN2);
+      FN(load_q)(h, i, j - 2, qN3);
+      FN(slope)(h, qLocN, qN0, qN1, qN2, qN3, dqN);
+
+      FN(trace)(h, qLoc, dq, dq + 4, dtdx, dtdy, FACE_YMIN, qright);
+      FN(trace)(h, qLocN, dqN, dqN + 4, dtdx, dtdy, FACE_YMAX, qleft);
+      tmp = qleft[2]; qleft[2] = qleft[3]; qleft[3] = tmp;
+      tmp = qright[2]; qright[2] = qright[3]; qright[3] = tmp;
+      FN(solve)(h, qleft, qright, qgdnv, flux_y);
+      tmp = flux_y[2]; flux_y[2] = flux_y[3]; flux_y[3] = tmp;
+      for (int v = 0; v < NBVAR; ++v) AT(h->fy, i, j, v + 1) = flux_y[v] * dtdy;
+    }
+}
+
+/* updateHydro: serial_ref/m_run.f90:489-518 (interior only) */
+static void FN(update_hydro)(FN(run) * h, const R *din, R *dout) {
+  const int gw = 2;
+  for (int j = gw + 1; j <= h->jsize - gw; ++j)
+    for (int i = gw + 1; i <= h->isize - gw; ++i)
+      for (int v = 1; v <= NBVAR; ++v) {
+        R flux_tot = AT(h->fx, i, j, v) - AT(h->fx, i + 1, j, v);
+        flux_tot = AT(h->fy, i, j, v) - AT(h->fy, i, j + 1, v) + flux_tot;
+        AT(dout, i, j, v) = AT(din, i, j, v) + flux_tot;
+      }
+}
+
+/* godunov_unsplit + godunov_unsplit_cpu: serial_ref/m_run.f90:187-243 */
+void FN(oracle_godunov)(void *hv, int nStep, double dt) {
+  FN(run) *h = (FN(run) *)hv;
+  R *din = (nStep % 2 == 0) ? h->u : h->u2;
+  R *dout = (nStep % 2 == 0) ? h->u2 : h->u;
+  FN(oracle_make_boundaries)(h, (nStep % 2 == 0) ? 0 : 1);
+  FN(convert_to_primitives)(h, din);
+  FN(compute_and_store_fluxes)(h, (R)dt);
+  FN(update_hydro)(h, din, dout);
+}
+
+/* main program loop: serial_ref/main.f90:36-76 (without output).  Returns the
+ * step count; dts[n] receives the dt used by step n when non-NULL. */
+int FN(oracle_run)(void *hv, int nStepmax, double tEnd, double *dts) {
+  FN(run) *h = (FN(run) *)hv;
+  R t = (R)0, dt;
+  int nStep = 0;
+  dt = (R)FN(oracle_compute_dt)(h, nStep % 2);
+  FN(oracle_make_boundaries)(h, 0);
+  while (t < (R)tEnd && nStep < nStepmax) {
+    dt = (R)FN(oracle_compute_dt)(h, nStep % 2);
+    if (dts) dts[nStep] = (double)dt;
+    FN(oracle_godunov)(h, nStep, (double)dt);
+    nStep = nStep + 1;
+  }
+  return nStep;
+}
+
+static R *FN(pick)(FN(run) * h, int which) {
+  switch (which) {
+    case ORACLE_U: return h->u;
+    case ORACLE_U2: return h->u2;
+    case ORACLE_Q: return h->q;
+    case ORACLE_FX: return h->fx;
+    case ORACLE_FY: return h->fy;
+    default: return NULL;
+  }
+}
+
+void FN(oracle_get)(void *hv, int which, double *out) {
+  FN(run) *h = (FN(run) *)hv;
+  R *a = FN(pick)(h, which);
+  size_t n = (size_t)h->isize * (size_t)h->jsize * NBVAR;
+  for (size_t k = 0; k < n; ++k) out[k] = (double)a[k];
+}
+
+void FN(oracle_set)(void *hv, int which, const double *in) {
+  FN(run) *h = (FN(run) *)hv;
+  R *a = FN(pick)(h, which);
+  size_t n = (size_t)h->isize * (size_t)h->jsize * NBVAR;
+  for (size_t k = 0; k < n; ++k) a[k] = (R)in[k];
+}
+
+/* Pointwise known-answer entry: same layouts as ref_kat.f90. */
+void FN(oracle_kat)(void *hv, int func, int n, const double *x, double *y) {
+  FN(run) *h = (FN(run) *)hv;
+  for (int k = 0; k < n; ++k) {
+    R a[20], o[8] = {0}, c = 0;
+    int nin = (func == ORACLE_KAT_PRIM) ? 4 : (func == ORACLE_KAT_SLOPE) ? 20
+            : (func == ORACLE_KAT_TRACE) ? 15 : (func == ORACLE_KAT_CMPFLX) ? 4
+            : (func == ORACLE_KAT_EOS) ? 2 : 8;
+    const double *xk = x + (size_t)k * (size_t)nin;
+    for (int m = 0; m < nin && m < 20; ++m) a[m] = (R)xk[m];
+    switch (func) {
+      case ORACLE_KAT_PRIM: {
+        FN(prim)(h, a, o, &c);
+        double *yk = y + (size_t)k * 5;
+        for (int m = 0; m < 4; ++m) yk[m] = (double)o[m];
+        yk[4] = (double)c;
+      } break;
+      case ORACLE_KAT_SLOPE: {
+        FN(slope)(h, a, a + 4, a + 8, a + 12, a + 16, o);
+        double *yk = y + (size_t)k * 8;
+        for (int m = 0; m < 8; ++m) yk[m] = (double)o[m];
+      } break;
+      case ORACLE_KAT_TRACE: {
+        FN(trace)(h, a, a + 4, a + 8, a[12], a[13], (int)xk[14], o);
+        double *yk = y + (size_t)k * 4;
+        for (int m = 0; m < 4; ++m) yk[m] = (double)o[m];
+      } break;
+      case ORACLE_KAT_RIEMANN:
+      case ORACLE_KAT_HLLC: {
+        R g[4];
+        if (func == ORACLE_KAT_RIEMANN)
+          (void)FN(riemann)(h, a, a + 4, g, o);
+        else
+          FN(hllc)(h, a, a + 4, g, o);
+        double *yk = y + (size_t)k * 8;
+        for (int m = 0; m < 4; ++m) yk[m] = (double)g[m];
+        for (int m = 0; m < 4; ++m) yk[4 + m] = (double)o[m];
+      } break;
+      case ORACLE_KAT_CMPFLX: {
+        FN(cmpflx)(h, a, o);
+        double *yk = y + (size_t)k * 4;
+        for (int m = 0; m < 4; ++m) yk[m] = (double)o[m];
+      } break;
+      case ORACLE_KAT_EOS: {
+        R p;
+        FN(eos)(h, a[0], a[1], &p, &c);
+        y[(size_t)k * 2] = (double)p;
+        y[(size_t)k * 2 + 1] = (double)c;
+      } break;
+      case ORACLE_KAT_RIEMANN_ITERS: {
+        R g[4];
+        y[k] = (double)FN(riemann)(h, a, a + 4, g, o);
+      } break;
+      default: break;
+    }
+  }
+}
+
+#undef AT
+#undef MAX2
+#undef MIN2
+#undef ABS
+#undef NBVAR

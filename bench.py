@@ -1,0 +1,252 @@
+"""Benchmark of the euler2d hot path on one MI355X.
+
+Default workload: 2048x2048 cells FP64, implementationVersion 0 (fused step),
+the diagonal-discontinuity problem (x+y<1: rho=1,p=1 else rho=0.125,p=0.1),
+gamma=1.4, CFL 0.8, minmod slopes, reflective walls.  A step is one pass of
+the reference's main-loop body over the grid (compute_dt + godunov_unsplit,
+serial_ref/main.f90:66-74); the state stays resident in HBM, dt stays on the
+device.  Prints one JSON line.
+
+  value     whole-job Mcell-updates/s = nx*ny*steps / wall time of the timed steps
+  roofline  the fused step kernel against HBM: algorithmic bytes per launch
+            (one read + one write of the 4 conserved variables per interior cell,
+            64 B/cell in FP64) / average kernel duration from hipEvents recorded
+            on the library's stream around each launch
+  cpu_baseline  the reference serial solver (oracle/_ref, compiled from
+            serial_ref/ with flang -O3, FP64 kind) on one core, 2048^2 for a few
+            steps; the oracle's C restatement if that build is absent
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = ROOT / "profiles" / "traffic.json"
+
+
+def _log(msg: str):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_run(e2d, nx, impl, prec, problem="diag"):
+    p = e2d.HydroParams(nx=nx, ny=nx, implementation_version=impl, precision=prec, problem=problem)
+    run = e2d.HydroRun(p)
+    run.init(problem, seed=20251202)
+    run.make_boundaries(0)
+    return run
+
+
+def time_steps(run, steps: int, warmup: int) -> tuple[float, float]:
+    """(seconds for `steps` timed steps, kernel-average ms from hipEvents)."""
+    run.run_steps(warmup)
+    run.synchronize()
+    t0 = time.perf_counter()
+    run.run_steps(steps)
+    run.synchronize()
+    wall = time.perf_counter() - t0
+    # kernel durations, in a separate pass so event records do not perturb the timed one
+    run.enable_timers(True)
+    run.get_timers()
+    run.run_steps(min(steps, 200))
+    t = run.get_timers()
+    run.enable_timers(False)
+    ms, n = t["godunov"]
+    return wall, ms / max(n, 1)
+
+
+def cpu_baseline(nx: int, steps: int) -> dict:
+    """Reference serial solver, one core, nx^2 for `steps` steps (bounded sample)."""
+    ref = ROOT / "oracle" / "_ref" / "fp64" / "ref_driver"
+    cores = 1
+    if ref.exists():
+        with tempfile.TemporaryDirectory() as td:
+            nml = pathlib.Path(td) / "bench.nml"
+            nml.write_text(
+                f"&RUN\ntEnd=10.0\nnStepmax={steps}\nnOutput=100000\n/\n"
+                f"&MESH\nnx={nx}\nny={nx}\nboundary_type_xmin=1\nboundary_type_xmax=1\n"
+                "boundary_type_ymin=1\nboundary_type_ymax=1\n/\n"
+                "&HYDRO\ngamma0=1.4\ncfl=0.8\nsmallr=1e-7\nsmallc=1e-7\nniter_riemann=10\n"
+                "iorder=2\nslope_type=1\n/\n"
+                "&ORACLE\ninitial_condition='diag'\ntiming=.true.\n/\n")
+            cmd = [str(ref), str(nml)]
+            if shutil_which("taskset"):
+                cmd = ["taskset", "-c", "0"] + cmd
+            out = subprocess.run(cmd, capture_output=True, text=True, check=True, timeout=600).stdout
+        m = re.search(r"wall_s=\s*([0-9.Ee+-]+)\s+mcell_per_s=\s*([0-9.Ee+-]+)", out)
+        return {"value": float(m.group(2)), "unit": "Mcell-updates/s", "cores": cores,
+                "kind": "reference",
+                "sample": f"{nx}x{nx} FP64, {steps} steps of the reference serial solver "
+                          f"(serial_ref/ built with flang -O3, FP64 kind), wall {float(m.group(1)):.1f} s"}
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_lib import IC_DIAG, Oracle
+    o = Oracle(nx)
+    o.init(IC_DIAG)
+    t0 = time.perf_counter()
+    o.run(steps)
+    wall = time.perf_counter() - t0
+    return {"value": nx * nx * steps / wall / 1e6, "unit": "Mcell-updates/s", "cores": cores,
+            "kind": "port", "sample": f"{nx}x{nx} FP64, {steps} steps of the oracle C restatement, "
+                                      f"wall {wall:.1f} s"}
+
+
+def shutil_which(name):
+    import shutil
+    return shutil.which(name)
+
+
+def variant(e2d, nx, impl, prec, steps, warmup):
+    run = make_run(e2d, nx, impl, prec)
+    try:
+        wall, kms = time_steps(run, steps, warmup)
+    finally:
+        run.close()
+    return {"mcell_per_s": round(nx * nx * steps / wall / 1e6, 1), "steps": steps,
+            "ms_per_step": round(1e3 * wall / steps, 4), "kernel_ms": round(kms, 4)}
+
+
+def precision_error(e2d, nx, steps):
+    """FP32 variants against the FP64 run on the bumps problem: relative L1 and
+    max-abs error per conserved variable after `steps` steps."""
+    fields = {}
+    for prec in (e2d.PREC_FP64, e2d.PREC_FP32, e2d.PREC_MIXED):
+        run = make_run(e2d, nx, 0, prec, problem="bumps")
+        run.run_steps(steps)
+        fields[prec] = run.download(run.current())[2:nx + 2, 2:nx + 2, :]
+        run.close()
+    ref = fields[e2d.PREC_FP64]
+    out = {}
+    for prec, name in ((e2d.PREC_FP32, "fp32"), (e2d.PREC_MIXED, "mixed")):
+        d = fields[prec] - ref
+        out[name] = {
+            "rel_l1": [float(np.abs(d[..., v]).sum() / np.abs(ref[..., v]).sum()) for v in range(4)],
+            "linf": [float(np.abs(d[..., v]).max()) for v in range(4)],
+        }
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=2048)
+    ap.add_argument("--impl", type=int, default=0)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "mixed"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (10 times) and exit; "
+                         "its FETCH_SIZE/WRITE_SIZE counters calibrate the HBM byte counters")
+    args = ap.parse_args(argv)
+
+    if args.calibrate_copy_mib:
+        import torch
+        n = args.calibrate_copy_mib * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 10}))
+        return 0
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU; a step does not shard across GPUs here "
+              "(the halo-exchange design is described in DESIGN.md). Use --gpus 1.", file=sys.stderr)
+        return 2
+
+    import euler2d_cudafortran_amd as e2d
+    e2d.load_library()
+    if not e2d.device_available():
+        print("bench.py: no HIP device", file=sys.stderr)
+        return 1
+    prec = e2d.PRECISIONS[args.precision]
+    nx = args.nx
+
+    _log(f"[bench] {nx}x{nx} {args.precision} impl {args.impl}: {args.warmup} warmup + {args.steps} steps")
+    run = make_run(e2d, nx, args.impl, prec)
+    wall, kernel_ms = time_steps(run, args.steps, args.warmup)
+    dt_final = run.get_dt()
+    run.close()
+    cells = nx * nx
+    value = cells * args.steps / wall / 1e6
+    ms_per_step = 1e3 * wall / args.steps
+    state_bytes = 8 if prec == e2d.PREC_FP64 else 4
+    alg_bytes = cells * 4 * state_bytes * 2  # read + write of 4 conserved variables
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    traffic = None
+    if TRAFFIC_FILE.exists():
+        tr = json.loads(TRAFFIC_FILE.read_text())
+        key = f"{nx}_{args.precision}_impl{args.impl}"
+        if key in tr:
+            traffic = tr[key]["bytes_per_launch"]
+    _log(f"[bench] {value:.1f} Mcell/s, {ms_per_step:.4f} ms/step, kernel {kernel_ms:.4f} ms, "
+         f"{achieved:.1f} GB/s algorithmic")
+
+    variants = {}
+    if not args.no_variants:
+        for name, (vnx, impl, vprec, vsteps) in {
+            "fp64_materialised_2048": (2048, 1, e2d.PREC_FP64, 200),
+            "fp32_fused_2048": (2048, 0, e2d.PREC_FP32, 500),
+            "mixed_fused_2048": (2048, 0, e2d.PREC_MIXED, 500),
+            "fp64_fused_4096": (4096, 0, e2d.PREC_FP64, 200),
+            "fp32_fused_4096": (4096, 0, e2d.PREC_FP32, 500),
+            "mixed_fused_4096": (4096, 0, e2d.PREC_MIXED, 500),
+        }.items():
+            _log(f"[bench] variant {name}")
+            variants[name] = variant(e2d, vnx, impl, vprec, vsteps, 10)
+        _log("[bench] precision error, bumps problem 4096^2, 500 steps")
+        variants["precision_error_bumps_4096_500steps_vs_fp64"] = precision_error(e2d, 4096, 500)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        _log(f"[bench] cpu baseline {nx}x{nx}, {args.cpu_steps} steps")
+        cpu = cpu_baseline(nx, args.cpu_steps)
+
+    line = {
+        "metric": "Mcell-updates/s at 2048^2 FP64 (fused step), %HBM peak",
+        "value": round(value, 2),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32", "mixed": "f32+f64"}[args.precision],
+        "data": "synthetic",
+        "config": {"workload": f"euler2d {nx}x{nx} {args.precision}, implementationVersion {args.impl}, "
+                               "diagonal discontinuity, gamma 1.4, CFL 0.8, minmod, reflective walls",
+                   "nx": nx, "ny": nx, "precision": args.precision,
+                   "implementation_version": args.impl,
+                   "mcell_updates_per_s_reference_formula": round((nx + 4) ** 2 * args.steps / wall / 1e6, 2),
+                   "dt_after_run": dt_final},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_step_fused" if args.impl == 0 else "materialised step (5 kernels)",
+                     "kernel_avg_ms": round(kernel_ms, 5),
+                     "algorithmic_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+        "variants": variants,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

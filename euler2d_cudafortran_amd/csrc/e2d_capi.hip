@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "../../include/euler2d_hip.h"
 #include "e2d_internal.hpp"
@@ -75,6 +76,16 @@ struct e2d_ctx {
   double invdt = 0.0;
   int dev_slot = -1;
   bool stages_valid = false;
+  // GPU timers: pending event pairs per category, accumulated on read
+  struct Pending {
+    int cat;
+    hipEvent_t start, stop;
+  };
+  bool timers_on = false;
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
+  double timer_ms[E2D_NUM_TIMERS] = {0, 0, 0};
+  int64_t timer_count[E2D_NUM_TIMERS] = {0, 0, 0};
 };
 
 namespace {
@@ -106,7 +117,53 @@ double dt_of(const e2d_params& p, double invdt) {
   return p.cfl / invdt;
 }
 
+hipEvent_t take_event(e2d_ctx* c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Launch `fn` on the context stream, bracketed by events when timers are on.
+template <class F>
+hipError_t timed(e2d_ctx* c, int cat, F&& fn) {
+  if (!c->timers_on) return fn();
+  hipEvent_t a = take_event(c), b = take_event(c);
+  if (!a || !b) return hipErrorOutOfMemory;
+  hipError_t e = hipEventRecord(a, c->stream);
+  if (e == hipSuccess) e = fn();
+  if (e == hipSuccess) e = hipEventRecord(b, c->stream);
+  c->pending.push_back({cat, a, b});
+  return e;
+}
+
+int collect_timers(e2d_ctx* c) {
+  if (c->pending.empty()) return E2D_OK;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (auto& pe : c->pending) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, pe.start, pe.stop));
+    c->timer_ms[pe.cat] += ms;
+    c->timer_count[pe.cat] += 1;
+    c->event_pool.push_back(pe.start);
+    c->event_pool.push_back(pe.stop);
+  }
+  c->pending.clear();
+  return E2D_OK;
+}
+
 void free_all(e2d_ctx* c) {
+  for (auto& pe : c->pending) {
+    (void)hipEventDestroy(pe.start);
+    (void)hipEventDestroy(pe.stop);
+  }
+  c->pending.clear();
+  for (auto e : c->event_pool) (void)hipEventDestroy(e);
+  c->event_pool.clear();
   for (auto& u : c->b.u)
     if (u) (void)hipFree(u), u = nullptr;
   for (auto& s : c->b.stage)
@@ -254,7 +311,7 @@ int e2d_compute_dt(e2d_ctx* c, int useU, double* dt) {
   if (int rc = pull_invdt(c)) return rc;
   const int buf = (useU == 0) ? 0 : 1;
   HIP_TRY(e2d::launch_fill(c->b.scal + 2, 1, c->invdt, c->stream));
-  HIP_TRY(e2d::launch_max_invdt(c->p, c->b, buf, 2, c->stream));
+  HIP_TRY(timed(c, E2D_TIMER_DT, [&] { return e2d::launch_max_invdt(c->p, c->b, buf, 2, c->stream); }));
   HIP_TRY(hipMemcpyAsync(&c->invdt, c->b.scal + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   *dt = dt_of(c->p, c->invdt);
@@ -264,7 +321,7 @@ int e2d_compute_dt(e2d_ctx* c, int useU, double* dt) {
 int e2d_make_boundaries(e2d_ctx* c, int buf) {
   if (int rc = check_ctx(c)) return rc;
   if (int rc = check_buf(buf)) return rc;
-  HIP_TRY(e2d::launch_boundaries(c->p, c->b, buf, c->stream));
+  HIP_TRY(timed(c, E2D_TIMER_BOUNDARIES, [&] { return e2d::launch_boundaries(c->p, c->b, buf, c->stream); }));
   return E2D_OK;
 }
 
@@ -279,9 +336,12 @@ int e2d_godunov_unsplit(e2d_ctx* c, int nStep, double dt) {
   cfg.dt_host = dt;
   cfg.track_dt = false;
   if (c->p.implementation_version == E2D_IMPL_FUSED) {
-    HIP_TRY(e2d::launch_fused_step(c->p, c->b, cfg, c->stream));
+    HIP_TRY(timed(c, E2D_TIMER_GODUNOV, [&] { return e2d::launch_fused_step(c->p, c->b, cfg, c->stream); }));
   } else {
-    HIP_TRY(e2d::launch_materialised_step(c->p, c->b, cfg, c->stream));
+    HIP_TRY(timed(c, E2D_TIMER_BOUNDARIES,
+                  [&] { return e2d::launch_boundaries(c->p, c->b, cfg.buf_in, c->stream); }));
+    HIP_TRY(timed(c, E2D_TIMER_GODUNOV,
+                  [&] { return e2d::launch_materialised_step(c->p, c->b, cfg, c->stream); }));
     c->stages_valid = true;
   }
   return E2D_OK;
@@ -298,7 +358,7 @@ int e2d_run_steps(e2d_ctx* c, int nsteps, int* nStep) {
     // compute_dt of the current input, folded into the running maximum
     k = 0;
     HIP_TRY(e2d::launch_fill(c->b.scal + k, 1, c->invdt, c->stream));
-    HIP_TRY(e2d::launch_max_invdt(c->p, c->b, n % 2, k, c->stream));
+    HIP_TRY(timed(c, E2D_TIMER_DT, [&] { return e2d::launch_max_invdt(c->p, c->b, n % 2, k, c->stream); }));
   }
   e2d::StepCfg cfg;
   cfg.dt_from_device = true;
@@ -312,7 +372,7 @@ int e2d_run_steps(e2d_ctx* c, int nsteps, int* nStep) {
       cfg.slot_cur = k;
       cfg.slot_next = 1 - k;
       cfg.track_dt = true;
-      HIP_TRY(e2d::launch_fused_step(c->p, c->b, cfg, c->stream));
+      HIP_TRY(timed(c, E2D_TIMER_GODUNOV, [&] { return e2d::launch_fused_step(c->p, c->b, cfg, c->stream); }));
       k = 1 - k;
     }
   } else {
@@ -320,8 +380,11 @@ int e2d_run_steps(e2d_ctx* c, int nsteps, int* nStep) {
       cfg.buf_in = n % 2;
       cfg.buf_out = 1 - cfg.buf_in;
       cfg.slot_cur = k;
-      HIP_TRY(e2d::launch_materialised_step(c->p, c->b, cfg, c->stream));
-      HIP_TRY(e2d::launch_max_invdt(c->p, c->b, cfg.buf_out, k, c->stream));
+      HIP_TRY(timed(c, E2D_TIMER_BOUNDARIES,
+                    [&] { return e2d::launch_boundaries(c->p, c->b, cfg.buf_in, c->stream); }));
+      HIP_TRY(timed(c, E2D_TIMER_GODUNOV,
+                    [&] { return e2d::launch_materialised_step(c->p, c->b, cfg, c->stream); }));
+      HIP_TRY(timed(c, E2D_TIMER_DT, [&] { return e2d::launch_max_invdt(c->p, c->b, cfg.buf_out, k, c->stream); }));
     }
     c->stages_valid = true;
   }
@@ -368,6 +431,25 @@ int e2d_sums(e2d_ctx* c, int buf, double out[4]) {
 int e2d_synchronize(e2d_ctx* c) {
   if (int rc = check_ctx(c)) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
+  return E2D_OK;
+}
+
+int e2d_enable_timers(e2d_ctx* c, int on) {
+  if (int rc = check_ctx(c)) return rc;
+  c->timers_on = (on != 0);
+  return E2D_OK;
+}
+
+int e2d_get_timers(e2d_ctx* c, double ms[E2D_NUM_TIMERS], int64_t count[E2D_NUM_TIMERS]) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!ms || !count) return fail(E2D_ERR_ARG, "output array is NULL");
+  if (int rc = collect_timers(c)) return rc;
+  for (int k = 0; k < E2D_NUM_TIMERS; ++k) {
+    ms[k] = c->timer_ms[k];
+    count[k] = c->timer_count[k];
+    c->timer_ms[k] = 0.0;
+    c->timer_count[k] = 0;
+  }
   return E2D_OK;
 }
 

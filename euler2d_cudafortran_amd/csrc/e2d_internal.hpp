@@ -40,6 +40,7 @@ size_t state_elem_bytes(int precision);   // storage type of the state
 size_t stage_elem_bytes(int precision, int stage);
 
 hipError_t launch_fused_step(const e2d_params& p, const DevBuffers& b, const StepCfg& c, hipStream_t s);
+// prim .. update of the materialised version (boundaries launched separately)
 hipError_t launch_materialised_step(const e2d_params& p, const DevBuffers& b, const StepCfg& c,
                                     hipStream_t s);
 hipError_t launch_boundaries(const e2d_params& p, const DevBuffers& b, int buf, hipStream_t s);

@@ -13,13 +13,15 @@
 // them.  The update epilogue also folds the CFL rate of the new state into the
 // running invDt of the next step (compute_dt, serial_ref/m_run.f90:248-312).
 //
-// Materialised step (implementationVersion 1): boundaries, primitive
-// variables, slopes, face states, fluxes and update as separate kernels with
+// Materialised step (implementationVersion 1): boundaries (launched by the
+// caller), primitive variables, slopes, face states, fluxes and update as
+// separate kernels with
 // every stage stored in HBM (cuf/m_HydroRun_gpu.cuf:393-418 structure, with
 // serial_ref arithmetic).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 
 #include "e2d_internal.hpp"
 #include "e2d_numerics.hpp"
@@ -27,17 +29,19 @@
 namespace e2d {
 
 // ---------------------------------------------------------------------------
-// precision policies: storage S, flux arithmetic T, update / dt type U, math M
-template <class S_, class T_, class U_, class M_>
+// precision policies: storage S, flux arithmetic T, update / dt type U, and
+// the division / square-root policies M (for T) and MU (for U)
+template <class S_, class T_, class U_, class M_, class MU_>
 struct Prec {
   using S = S_;
   using T = T_;
   using U = U_;
   using M = M_;
+  using MU = MU_;
 };
-using P64 = Prec<double, double, double, ExactMath>;
-using P32 = Prec<float, float, float, ExactMath>;
-using PMix = Prec<float, float, double, FastMath>;
+using P64 = Prec<double, double, double, RangedMath, RangedMath>;
+using P32 = Prec<float, float, float, ExactMath, ExactMath>;
+using PMix = Prec<float, float, double, FastMath, RangedMath>;
 
 // Derived parameters in precision T, as initHydroParameters computes them
 // (serial_ref/m_run.f90:111-124) plus the cmpflx constant entho (m_utils.f90:521).
@@ -189,8 +193,17 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nb) {
 // ---------------------------------------------------------------------------
 // fused step
 
-constexpr int kStripOut = 60;     // output columns per wavefront
-constexpr int kWavesPerBlock = 4;
+constexpr int kStripOut = 60;  // output columns per wavefront
+#ifndef E2D_WAVES_PER_BLOCK
+#define E2D_WAVES_PER_BLOCK 4
+#endif
+constexpr int kWavesPerBlock = E2D_WAVES_PER_BLOCK;  // independent wavefronts per workgroup
+// optional register cap for the fused kernel (minimum resident waves per SIMD)
+#if defined(E2D_FUSED_MIN_WAVES) && E2D_FUSED_MIN_WAVES > 0
+#define E2D_FUSED_ATTR __attribute__((amdgpu_waves_per_eu(E2D_FUSED_MIN_WAVES)))
+#else
+#define E2D_FUSED_ATTR
+#endif
 
 template <class P>
 struct StepArgs {
@@ -207,14 +220,13 @@ struct StepArgs {
   int seg_groups; // ceil(nsegs / kWavesPerBlock)
 };
 
-// Load cell (ci, r) of the input, boundary-filled, as T; write ghost values back.
+// Load cell (ci, r) of the input, boundary-filled (storage type); write the
+// ghost value back so the input buffer ends up as make_boundaries leaves it.
 template <class P>
 __device__ __forceinline__ void load_cell(const StepArgs<P>& a, int ci, bool col_ok, int r,
-                                          typename P::T u[4]) {
-  using T = typename P::T;
+                                          typename P::S raw[4]) {
   using S = typename P::S;
   const Src s = ghost_source(a.g, ci, r);
-  S raw[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const S x = a.uin[at(a.g, s.i, s.j, v)];
@@ -224,12 +236,20 @@ __device__ __forceinline__ void load_cell(const StepArgs<P>& a, int ci, bool col
 #pragma unroll
     for (int v = 0; v < 4; ++v) a.uin[at(a.g, ci, r, v)] = raw[v];
   }
+}
+
+template <class P>
+__device__ __forceinline__ void prim_of(const Hydro<typename P::T>& h, const typename P::S raw[4],
+                                        typename P::T q[4]) {
+  using T = typename P::T;
+  T u[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) u[v] = static_cast<T>(raw[v]);
+  cons_to_prim<T, typename P::M>(h, u, q);
 }
 
 template <class P, int SOLVER>
-__global__ __launch_bounds__(256) void k_step_fused(const StepArgs<P> a) {
+__global__ __launch_bounds__(64 * kWavesPerBlock) E2D_FUSED_ATTR void k_step_fused(const StepArgs<P> a) {
   using T = typename P::T;
   using U = typename P::U;
   using S = typename P::S;
@@ -265,11 +285,13 @@ __global__ __launch_bounds__(256) void k_step_fused(const StepArgs<P> a) {
   const T dtdy = static_cast<T>(dtdyU);
   const Hydro<T>& h = a.hT;
 
-  T qa[4], qb[4], qc[4], u[4];
-  load_cell<P>(a, cl, col_ok, j0 - 2, u);
-  cons_to_prim<T, M>(h, u, qa);
-  load_cell<P>(a, cl, col_ok, j0 - 1, u);
-  cons_to_prim<T, M>(h, u, qb);
+  T qa[4], qb[4], qc[4];
+  S raw[4], raw_next[4];
+  load_cell<P>(a, cl, col_ok, j0 - 2, raw);
+  prim_of<P>(h, raw, qa);
+  load_cell<P>(a, cl, col_ok, j0 - 1, raw);
+  prim_of<P>(h, raw, qb);
+  load_cell<P>(a, cl, col_ok, j0, raw_next);
 
   T ymax_prev[4];
   U fy_prev[4] = {U(0), U(0), U(0), U(0)}, fx_prev[4] = {U(0), U(0), U(0), U(0)};
@@ -277,8 +299,19 @@ __global__ __launch_bounds__(256) void k_step_fused(const StepArgs<P> a) {
   U rate_max = U(0);
 
   for (int r = j0 - 1; r <= j1; ++r) {
-    load_cell<P>(a, cl, col_ok, r + 1, u);
-    cons_to_prim<T, M>(h, u, qc);
+    // software pipeline: row r+1 was fetched one iteration ago; fetch row r+2
+    // now and the input of the cell updated this iteration, so both loads are
+    // in flight during the Riemann solves
+#pragma unroll
+    for (int v = 0; v < 4; ++v) raw[v] = raw_next[v];
+    if (r + 2 <= j1 + 1) load_cell<P>(a, cl, col_ok, r + 2, raw_next);
+    S uold[4];
+    const bool do_update = (r >= j0 + 1);
+    if (do_update && out_lane) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) uold[v] = a.uin[at(g, ci, r - 1, v)];
+    }
+    prim_of<P>(h, raw, qc);
 
     // slopes and trace sources of row r
     T qxm[4], qxp[4], dqx[4], dqy[4], s[4];
@@ -313,7 +346,7 @@ __global__ __launch_bounds__(256) void k_step_fused(const StepArgs<P> a) {
     }
 
     // conservative update of row r-1 (updateHydro, serial_ref/m_run.f90:504-513)
-    if (r >= j0 + 1) {
+    if (do_update) {
       U fxr[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) fxr[v] = from_right(fx_prev[v]);
@@ -322,15 +355,14 @@ __global__ __launch_bounds__(256) void k_step_fused(const StepArgs<P> a) {
         U un[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const U uo = static_cast<U>(a.uin[at(g, ci, jr, v)]);
           const U flux_tot = (fy_prev[v] - fy_cur[v]) + (fx_prev[v] - fxr[v]);
-          const S sn = static_cast<S>(uo + flux_tot);
+          const S sn = static_cast<S>(static_cast<U>(uold[v]) + flux_tot);
           a.uout[at(g, ci, jr, v)] = sn;
           un[v] = static_cast<U>(sn);
         }
         // CFL rate of the new state over the compute_dt window [2, n] (0-based)
         if (a.track_dt && ci <= g.nx && jr <= g.ny) {
-          const U rate = cfl_rate<U, ExactMath>(a.hU, un);
+          const U rate = cfl_rate<U, typename P::MU>(a.hU, un);
           rate_max = rate_max > rate ? rate_max : rate;
         }
       }
@@ -353,6 +385,38 @@ __global__ __launch_bounds__(256) void k_step_fused(const StepArgs<P> a) {
   }
 }
 
+// Rows per wavefront segment.  Each segment re-primes its rolling rows (two
+// extra row loads and primitive conversions, one extra row of y-faces), so
+// long segments are cheaper per row; short segments give more wavefronts to
+// hide latency and balance the data-dependent Newton work.  Measured on
+// MI355X (FP64, 3 resident waves per SIMD): about ny/170 rows is best
+// (12 at 2048^2, 24 at 4096^2); kernels with more resident waves per SIMD
+// want proportionally fewer rows.  E2D_FUSED_SEG_ROWS overrides the choice.
+static int choose_seg_rows(const void* kernel, int ny) {
+  if (const char* env = getenv("E2D_FUSED_SEG_ROWS")) {
+    const int r = atoi(env);
+    if (r >= 1) return r;
+  }
+  int blocks_per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, kernel, 64 * kWavesPerBlock, 0) !=
+          hipSuccess ||
+      blocks_per_cu <= 0)
+    blocks_per_cu = 12 / kWavesPerBlock;
+  const double waves_per_simd = blocks_per_cu * kWavesPerBlock / 4.0;
+  const int r = static_cast<int>(ny / 170.0 * 3.0 / waves_per_simd + 0.5);
+  return r < 8 ? 8 : (r > 32 ? 32 : r);
+}
+
+template <class P, int SOLVER>
+static int seg_rows_for(int ny) {
+  static int cached_ny = -1, cached = 16;
+  if (ny != cached_ny) {
+    cached = choose_seg_rows(reinterpret_cast<const void*>(k_step_fused<P, SOLVER>), ny);
+    cached_ny = ny;
+  }
+  return cached;
+}
+
 template <class P>
 static hipError_t fused_step(const e2d_params& p, const DevBuffers& b, const StepCfg& c, hipStream_t st) {
   StepArgs<P> a;
@@ -367,12 +431,13 @@ static hipError_t fused_step(const e2d_params& p, const DevBuffers& b, const Ste
   a.track_dt = c.track_dt ? 1 : 0;
   a.slot_next = c.slot_next;
   a.dt_host = c.dt_host;
-  a.seg_rows = 16;
+  const bool hllc = p.riemann_solver == E2D_RIEMANN_HLLC;
+  a.seg_rows = hllc ? seg_rows_for<P, 2>(p.ny) : seg_rows_for<P, 0>(p.ny);
   a.nsegs = (p.ny + a.seg_rows - 1) / a.seg_rows;
   a.seg_groups = (a.nsegs + kWavesPerBlock - 1) / kWavesPerBlock;
   const int nstrips = (p.nx + kStripOut - 1) / kStripOut;
   const int nblocks = nstrips * a.seg_groups;
-  if (p.riemann_solver == E2D_RIEMANN_HLLC)
+  if (hllc)
     hipLaunchKernelGGL((k_step_fused<P, 2>), dim3(nblocks), dim3(64 * kWavesPerBlock), 0, st, a);
   else
     hipLaunchKernelGGL((k_step_fused<P, 0>), dim3(nblocks), dim3(64 * kWavesPerBlock), 0, st, a);
@@ -452,7 +517,7 @@ __global__ __launch_bounds__(256) void k_max_invdt(const Hydro<typename P::U> h,
     U c[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) c[v] = static_cast<U>(u[at(g, i, j, v)]);
-    const U r = cfl_rate<U, ExactMath>(h, c);
+    const U r = cfl_rate<U, typename P::MU>(h, c);
     m = m > r ? m : r;
   }
   double md = wave_max(static_cast<double>(m));
@@ -663,8 +728,6 @@ static hipError_t materialised_step(const e2d_params& p, const DevBuffers& b, co
   a.slot_cur = c.slot_cur;
   a.dt_host = c.dt_host;
   const int S_ = p.nx + 4, J_ = p.ny + 4;
-  hipError_t e = boundaries<P>(p, b, c.buf_in, st);
-  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_prim<P>, dim3((S_ + 63) / 64, (J_ + 3) / 4), dim3(256), 0, st, a);
   hipLaunchKernelGGL(k_slopes<P>, dim3((S_ - 2 + 63) / 64, (J_ - 2 + 3) / 4), dim3(256), 0, st, a);
   hipLaunchKernelGGL(k_trace<P>, dim3((S_ - 2 + 63) / 64, (J_ - 2 + 3) / 4), dim3(256), 0, st, a);
@@ -926,7 +989,7 @@ __global__ __launch_bounds__(256) void k_pointwise(const Hydro<typename P::T> h,
       if (func == E2D_FN_RIEMANN)
         riemann_two_shock<T, M>(h, a, a + 4, g, o);
       else
-        riemann_hllc<T, M>(h, a, a + 4, g, o);
+        riemann_hllc<T, ExactMath>(h, a, a + 4, g, o);
       for (int m = 0; m < 4; ++m) y[8 * k + m] = g[m];
       for (int m = 0; m < 4; ++m) y[8 * k + 4 + m] = o[m];
     } break;

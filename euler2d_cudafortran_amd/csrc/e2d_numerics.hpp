@@ -41,6 +41,44 @@ struct FastMath {
   static __device__ __forceinline__ float sqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
 };
 
+// RangedMath (FP64): correctly rounded division and square root for finite,
+// nonzero operands whose binary exponents stay far from the double limits
+// (|log2| below ~700; the solver's values span about 1e-22..1e9).  These are
+// the Newton-Raphson (division) and Goldschmidt (square root) cores of the
+// IEEE sequences the compiler emits for a / b and sqrt(x) on gfx950, without
+// the range scaling (v_div_scale, ldexp by 2^256) and special-value fixup
+// (v_div_fixup, class tests) that act only outside that range.  Inside it the
+// results are the IEEE results bit for bit, except that a zero quotient is +0
+// whatever the sign of the numerator; a zero radicand returns zero.  (FP32
+// values of the solver come close enough to the float limits for the scaling
+// to matter, so the FP32 path keeps ExactMath.)
+struct RangedMath {
+  static __device__ __forceinline__ double div(double a, double b) {
+    const double nb = -b;
+    double r = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(nb, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(nb, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    const double q = a * r;
+    const double rem = __builtin_fma(nb, q, a);
+    return __builtin_fma(rem, r, q);
+  }
+  static __device__ __forceinline__ double sqrt(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double hh = y * 0.5;
+    const double r = __builtin_fma(-hh, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    hh = __builtin_fma(hh, r, hh);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, hh, g);
+    d = __builtin_fma(-g, g, x);
+    const double s = __builtin_fma(d, hh, g);
+    return x == 0.0 ? x : s;
+  }
+};
+
 // Fortran max/min as the reference compiler lowers them (first operand wins
 // only when strictly greater / smaller; this fixes the sign of zero results).
 template <class T>
@@ -281,10 +319,12 @@ __device__ __forceinline__ void riemann_hllc(const Hydro<T>& h, const T ql[4], c
 }
 
 // Solver selection at compile time (riemannSolverType, serial_ref/m_constants.f90:21-23).
+// HLLC divides by wave-speed differences that can vanish, so it always keeps
+// IEEE division (the reference's inf/NaN behaviour included).
 template <class T, class M, int SOLVER>
 __device__ __forceinline__ void riemann(const Hydro<T>& h, const T ql[4], const T qr[4], T g[4], T f[4]) {
   if constexpr (SOLVER == 2)
-    riemann_hllc<T, M>(h, ql, qr, g, f);
+    riemann_hllc<T, ExactMath>(h, ql, qr, g, f);
   else
     riemann_two_shock<T, M>(h, ql, qr, g, f);
 }

@@ -75,6 +75,8 @@ _SIGS = {
     "e2d_sums": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     "e2d_download_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     "e2d_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "e2d_enable_timers": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "e2d_get_timers": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "e2d_device_buffer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
                                          ctypes.POINTER(ctypes.c_int64)]),
     "e2d_pointwise": (ctypes.c_int, [ctypes.POINTER(_Params), ctypes.c_int, ctypes.c_int,
@@ -331,6 +333,16 @@ class HydroRun:
 
     def synchronize(self):
         _check(self.lib.e2d_synchronize(self.ctx))
+
+    def enable_timers(self, on: bool = True):
+        _check(self.lib.e2d_enable_timers(self.ctx, 1 if on else 0))
+
+    def get_timers(self) -> dict:
+        """{'godunov'|'boundaries'|'dt': (milliseconds, launches)} since the last call."""
+        ms = np.zeros(3, dtype=np.float64)
+        cnt = np.zeros(3, dtype=np.int64)
+        _check(self.lib.e2d_get_timers(self.ctx, ms.ctypes.data, cnt.ctypes.data))
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(("godunov", "boundaries", "dt"))}
 
     def device_buffer(self, buf: int):
         p = ctypes.c_void_p()

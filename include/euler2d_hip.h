@@ -174,6 +174,20 @@ int e2d_download_stage(e2d_ctx *ctx, int stage, double *h);
 /* Wait for all work of the context. */
 int e2d_synchronize(e2d_ctx *ctx);
 
+/* GPU timers, replacing the CudaTimer event pairs of the reference
+ * (cuf/m_Monitoring_gpu.cuf:14-129; d_godunov_timer, d_boundaries_timer,
+ * cuf/m_HydroRun_gpu.cuf:365-451).  While enabled, every launch made by
+ * e2d_godunov_unsplit / e2d_run_steps / e2d_compute_dt is bracketed by hipEvents
+ * recorded on the context's stream.  e2d_get_timers synchronises, returns the
+ * accumulated milliseconds and launch counts per category since the last call,
+ * and resets them. */
+#define E2D_TIMER_GODUNOV 0    /* the step kernel(s) */
+#define E2D_TIMER_BOUNDARIES 1 /* ghost fill (a separate launch in version 1 only) */
+#define E2D_TIMER_DT 2         /* standalone compute_dt reductions */
+#define E2D_NUM_TIMERS 3
+int e2d_enable_timers(e2d_ctx *ctx, int on);
+int e2d_get_timers(e2d_ctx *ctx, double ms[E2D_NUM_TIMERS], int64_t count[E2D_NUM_TIMERS]);
+
 /* Device pointer and row pitch (in elements) of a state buffer; element
  * (i, j, v) (0-based) sits at ptr[(v * jsize + j) * pitch + i], in the
  * precision's storage type (double for FP64, float for FP32/MIXED). */

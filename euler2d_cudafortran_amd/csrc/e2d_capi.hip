@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/euler2d_hip.h"
+#include "e2d_adjoint.hpp"
 #include "e2d_internal.hpp"
 
 namespace {
@@ -86,6 +87,10 @@ struct e2d_ctx {
   std::vector<hipEvent_t> event_pool;
   double timer_ms[E2D_NUM_TIMERS] = {0, 0, 0};
   int64_t timer_count[E2D_NUM_TIMERS] = {0, 0, 0};
+  // tangent / adjoint scratch (allocated on first use)
+  void* dual[2] = {nullptr, nullptr};
+  e2d::AdjointScratch adj;
+  std::vector<void*> snapshots;
 };
 
 namespace {
@@ -156,7 +161,24 @@ int collect_timers(e2d_ctx* c) {
   return E2D_OK;
 }
 
+size_t state_bytes(const e2d_ctx* c) {
+  return static_cast<size_t>(c->b.plane) * 4 * e2d::state_elem_bytes(c->p.precision);
+}
+size_t field_bytes(const e2d_ctx* c) { return static_cast<size_t>(c->b.plane) * 4 * sizeof(double); }
+
+void free_adjoint(e2d_ctx* c) {
+  for (auto& d : c->dual)
+    if (d) (void)hipFree(d), d = nullptr;
+  double** arrs[] = {&c->adj.q, &c->adj.xminb, &c->adj.xmaxb, &c->adj.yminb, &c->adj.ymaxb,
+                     &c->adj.qb, &c->adj.dqxb, &c->adj.dqyb, &c->adj.ub};
+  for (double** a : arrs)
+    if (*a) (void)hipFree(*a), *a = nullptr;
+  for (void* sn : c->snapshots) (void)hipFree(sn);
+  c->snapshots.clear();
+}
+
 void free_all(e2d_ctx* c) {
+  free_adjoint(c);
   for (auto& pe : c->pending) {
     (void)hipEventDestroy(pe.start);
     (void)hipEventDestroy(pe.stop);
@@ -459,6 +481,215 @@ int e2d_device_buffer(e2d_ctx* c, int buf, void** dptr, int64_t* pitch) {
   if (!dptr || !pitch) return fail(E2D_ERR_ARG, "output pointer is NULL");
   *dptr = c->b.u[buf];
   *pitch = c->b.pitch;
+  return E2D_OK;
+}
+
+namespace {
+
+int check_linearisable(const e2d_ctx* c) {
+  if (c->p.precision != E2D_PREC_FP64)
+    return fail(E2D_ERR_STATE, "tangent and adjoint need precision 0 (fp64)");
+  if (c->p.riemann_solver != E2D_RIEMANN_APPROX)
+    return fail(E2D_ERR_STATE, "tangent and adjoint need the two-shock solver (riemann_solver 0)");
+  return E2D_OK;
+}
+
+int ensure_adjoint_scratch(e2d_ctx* c) {
+  if (c->adj.q) return E2D_OK;
+  double** arrs[] = {&c->adj.q, &c->adj.xminb, &c->adj.xmaxb, &c->adj.yminb, &c->adj.ymaxb,
+                     &c->adj.qb, &c->adj.dqxb, &c->adj.dqyb, &c->adj.ub};
+  for (double** a : arrs) {
+    if (hipMalloc(reinterpret_cast<void**>(a), field_bytes(c)) != hipSuccess) {
+      free_adjoint(c);
+      return fail(E2D_ERR_NOMEM, "adjoint scratch allocation failed");
+    }
+    // slots never written by a pass must read as zero
+    HIP_TRY(hipMemsetAsync(*a, 0, field_bytes(c), c->stream));
+  }
+  return E2D_OK;
+}
+
+int adjoint_step(e2d_ctx* c, int buf, double dt, const double* lam_in, double* lam_out) {
+  HIP_TRY(timed(c, E2D_TIMER_GODUNOV,
+                [&] { return e2d::launch_adjoint_step(c->p, c->b, buf, dt, lam_in, lam_out, c->adj, c->stream); }));
+  return E2D_OK;
+}
+
+// steps n -> n+1 with a given dt (the forward of godunov_unsplit)
+int forward_step(e2d_ctx* c, int n, double dt) {
+  e2d::StepCfg cfg;
+  cfg.buf_in = n % 2;
+  cfg.buf_out = 1 - cfg.buf_in;
+  cfg.dt_host = dt;
+  HIP_TRY(e2d::launch_fused_step(c->p, c->b, cfg, c->stream));
+  return E2D_OK;
+}
+
+// Binomial checkpointing (Griewank's treeverse).  beta(s, t) = C(s+t, s) steps
+// can be reversed with s snapshots and at most t forward sweeps per step.
+double beta(int s, int t) {
+  double r = 1.0;
+  for (int k = 1; k <= s; ++k) r = r * (t + k) / k;
+  return r;
+}
+
+struct Rollout {
+  e2d_ctx* c;
+  int n0;
+  const double* dts;
+  double* lam;
+  std::vector<int> free_slots;
+  int64_t nfwd = 0;
+
+  int buf_of(int k) const { return (n0 + k) % 2; }
+  int save(int slot, int k) {
+    HIP_TRY(hipMemcpyAsync(c->snapshots[slot], c->b.u[buf_of(k)], state_bytes(c), hipMemcpyDeviceToDevice,
+                           c->stream));
+    return E2D_OK;
+  }
+  int restore(int slot, int k) {
+    HIP_TRY(hipMemcpyAsync(c->b.u[buf_of(k)], c->snapshots[slot], state_bytes(c), hipMemcpyDeviceToDevice,
+                           c->stream));
+    return E2D_OK;
+  }
+  int advance(int from, int to) {
+    for (int k = from; k < to; ++k) {
+      if (int rc = forward_step(c, n0 + k, dts[k])) return rc;
+      ++nfwd;
+    }
+    return E2D_OK;
+  }
+  // lam <- J_a^T ... J_{b-1}^T lam, with the state of step a in snapshot `slot`
+  int reverse(int a, int b, int slot) {
+    const int l = b - a;
+    const int s = static_cast<int>(free_slots.size());
+    if (l == 1 || s == 0) {
+      for (int k = b - 1; k >= a; --k) {
+        if (int rc = restore(slot, a)) return rc;
+        if (int rc = advance(a, k)) return rc;
+        if (int rc = adjoint_step(c, buf_of(k), dts[k], lam, lam)) return rc;
+      }
+      return E2D_OK;
+    }
+    int t = 1;
+    while (beta(s, t) < l) ++t;
+    int m = static_cast<int>(l - beta(s - 1, t));
+    if (m < 1) m = 1;
+    if (m > l - 1) m = l - 1;
+    if (int rc = restore(slot, a)) return rc;
+    if (int rc = advance(a, a + m)) return rc;
+    const int mine = free_slots.back();
+    free_slots.pop_back();
+    if (int rc = save(mine, a + m)) return rc;
+    if (int rc = reverse(a + m, b, mine)) return rc;
+    free_slots.push_back(mine);
+    return reverse(a, a + m, slot);
+  }
+};
+
+}  // namespace
+
+int e2d_tangent_step(e2d_ctx* c, int buf, double dt, const double* v_in, double* v_out) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_buf(buf)) return rc;
+  if (int rc = check_linearisable(c)) return rc;
+  if (!v_in || !v_out) return fail(E2D_ERR_ARG, "field pointer is NULL");
+  if (!(dt > 0.0)) return fail(E2D_ERR_ARG, "dt must be > 0");
+  for (auto& d : c->dual)
+    if (!d && hipMalloc(&d, static_cast<size_t>(c->b.plane) * 4 * sizeof(e2d::Dual)) != hipSuccess)
+      return fail(E2D_ERR_NOMEM, "tangent scratch allocation failed");
+  HIP_TRY(timed(c, E2D_TIMER_GODUNOV, [&] {
+    return e2d::launch_tangent_step(c->p, c->b, buf, dt, v_in, v_out, c->dual[0], c->dual[1], c->stream);
+  }));
+  return E2D_OK;
+}
+
+int e2d_adjoint_step(e2d_ctx* c, int buf, double dt, const double* lam_in, double* lam_out) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_buf(buf)) return rc;
+  if (int rc = check_linearisable(c)) return rc;
+  if (!lam_in || !lam_out) return fail(E2D_ERR_ARG, "field pointer is NULL");
+  if (!(dt > 0.0)) return fail(E2D_ERR_ARG, "dt must be > 0");
+  if (int rc = ensure_adjoint_scratch(c)) return rc;
+  return adjoint_step(c, buf, dt, lam_in, lam_out);
+}
+
+int e2d_adjoint_rollout(e2d_ctx* c, int nStep0, int nsteps, const double* dts, int nsnap, const double* lam_n,
+                        double* lam_0, int64_t* n_forward) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_linearisable(c)) return rc;
+  if (nStep0 < 0 || nsteps < 1) return fail(E2D_ERR_ARG, "need nStep0 >= 0 and nsteps >= 1");
+  if (nsnap < 1) return fail(E2D_ERR_ARG, "need at least one snapshot");
+  if (!dts || !lam_n || !lam_0) return fail(E2D_ERR_ARG, "NULL argument");
+  for (int k = 0; k < nsteps; ++k)
+    if (!(dts[k] > 0.0)) return fail(E2D_ERR_ARG, "dts[%d] must be > 0", k);
+  if (int rc = ensure_adjoint_scratch(c)) return rc;
+  while (static_cast<int>(c->snapshots.size()) < nsnap) {
+    void* sn = nullptr;
+    if (hipMalloc(&sn, state_bytes(c)) != hipSuccess)
+      return fail(E2D_ERR_NOMEM, "snapshot %zu allocation failed", c->snapshots.size());
+    c->snapshots.push_back(sn);
+  }
+  if (lam_0 != lam_n)
+    HIP_TRY(hipMemcpyAsync(lam_0, lam_n, field_bytes(c), hipMemcpyDeviceToDevice, c->stream));
+  Rollout r{c, nStep0, dts, lam_0, {}, 0};
+  for (int k = nsnap - 1; k >= 1; --k) r.free_slots.push_back(k);
+  if (int rc = r.save(0, 0)) return rc;
+  if (int rc = r.reverse(0, nsteps, 0)) return rc;
+  if (int rc = r.restore(0, 0)) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (n_forward) *n_forward = r.nfwd;
+  return E2D_OK;
+}
+
+int e2d_field_alloc(e2d_ctx* c, double** d_field) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!d_field) return fail(E2D_ERR_ARG, "NULL argument");
+  *d_field = nullptr;
+  HIP_TRY(hipMalloc(reinterpret_cast<void**>(d_field), field_bytes(c)));
+  HIP_TRY(hipMemsetAsync(*d_field, 0, field_bytes(c), c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return E2D_OK;
+}
+
+int e2d_field_free(e2d_ctx* c, double* d_field) {
+  if (int rc = check_ctx(c)) return rc;
+  if (d_field) HIP_TRY(hipFree(d_field));
+  return E2D_OK;
+}
+
+int e2d_field_upload(e2d_ctx* c, double* d_field, const double* h) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!d_field || !h) return fail(E2D_ERR_ARG, "NULL argument");
+  if (!c->staging)
+    HIP_TRY(hipMalloc(reinterpret_cast<void**>(&c->staging), host_elems(c) * sizeof(double)));
+  HIP_TRY(hipMemcpyAsync(c->staging, h, host_elems(c) * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(e2d::launch_pack_field(c->p, c->b, c->staging, d_field, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return E2D_OK;
+}
+
+int e2d_field_download(e2d_ctx* c, const double* d_field, double* h) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!d_field || !h) return fail(E2D_ERR_ARG, "NULL argument");
+  return download_array(c, d_field, 8, h);
+}
+
+int e2d_copy_to_buffer(e2d_ctx* c, int buf, const void* d_src) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_buf(buf)) return rc;
+  if (!d_src) return fail(E2D_ERR_ARG, "NULL argument");
+  HIP_TRY(hipMemcpyAsync(c->b.u[buf], d_src, state_bytes(c), hipMemcpyDeviceToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return E2D_OK;
+}
+
+int e2d_copy_from_buffer(e2d_ctx* c, int buf, void* d_dst) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_buf(buf)) return rc;
+  if (!d_dst) return fail(E2D_ERR_ARG, "NULL argument");
+  HIP_TRY(hipMemcpyAsync(d_dst, c->b.u[buf], state_bytes(c), hipMemcpyDeviceToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return E2D_OK;
 }
 

@@ -1,0 +1,189 @@
+// Pieces shared by the step, tangent and adjoint kernels: precision
+// policies, derived parameters, grid description, the ghost-cell source map,
+// and wave64 cross-lane helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "e2d_adjoint.hpp"
+#include "e2d_internal.hpp"
+#include "e2d_numerics.hpp"
+
+namespace e2d {
+
+// ---------------------------------------------------------------------------
+// precision policies: storage S, flux arithmetic T, update / dt type U, and
+// the division / square-root policies M (for T) and MU (for U)
+template <class S_, class T_, class U_, class M_, class MU_>
+struct Prec {
+  using S = S_;
+  using T = T_;
+  using U = U_;
+  using M = M_;
+  using MU = MU_;
+};
+using P64 = Prec<double, double, double, RangedMath, RangedMath>;
+using P32 = Prec<float, float, float, ExactMath, ExactMath>;
+using PMix = Prec<float, float, double, FastMath, RangedMath>;
+
+// Derived parameters in precision T, as initHydroParameters computes them
+// (serial_ref/m_run.f90:111-124) plus the cmpflx constant entho (m_utils.f90:521).
+template <class T>
+inline Hydro<T> make_hydro(const e2d_params& p) {
+  Hydro<T> h;
+  h.gamma0 = static_cast<T>(p.gamma0);
+  h.cfl = static_cast<T>(p.cfl);
+  h.smallr = static_cast<T>(p.smallr);
+  h.smallc = static_cast<T>(p.smallc);
+  h.smallp = h.smallc * h.smallc / h.gamma0;
+  h.smallpp = h.smallr * h.smallp;
+  h.gamma6 = (h.gamma0 + T(1)) / (T(2) * h.gamma0);
+  h.entho = T(1) / (h.gamma0 - T(1));
+  h.dx = (T(1) - T(0)) / static_cast<T>(p.nx);
+  h.dy = (T(1) - T(0)) / static_cast<T>(p.ny);
+  h.conv_tol = static_cast<T>(1e-6f);
+  h.slope_type = p.slope_type;
+  h.niter = p.niter_riemann;
+  h.riemann_solver = p.riemann_solver;
+  return h;
+}
+
+struct Grid {
+  int nx, ny, isize, jsize;
+  int64_t pitch, plane;
+  int bc[4];  // xmin, xmax, ymin, ymax
+};
+
+inline Grid make_grid(const e2d_params& p, const DevBuffers& b) {
+  Grid g;
+  g.nx = p.nx;
+  g.ny = p.ny;
+  g.isize = p.nx + 4;
+  g.jsize = p.ny + 4;
+  g.pitch = b.pitch;
+  g.plane = b.plane;
+  g.bc[0] = p.boundary_type_xmin;
+  g.bc[1] = p.boundary_type_xmax;
+  g.bc[2] = p.boundary_type_ymin;
+  g.bc[3] = p.boundary_type_ymax;
+  return g;
+}
+
+// Where the value of cell (i, j) comes from once boundaries are filled.
+// Non-corner ghost cells map to an interior cell (reflective: mirror with the
+// normal momentum negated; absorbing: nearest interior cell; other: periodic),
+// everything else (interior cells and the never-filled corners) maps to itself.
+struct Src {
+  int i, j;
+  int flip;  // variable whose sign flips (2 = rho*u, 3 = rho*v), or -1
+  bool ghost;
+};
+
+__device__ __forceinline__ Src ghost_source(const Grid& g, int i, int j) {
+  Src s{i, j, -1, false};
+  const bool xg = (i < 2) || (i >= g.nx + 2);
+  const bool yg = (j < 2) || (j >= g.ny + 2);
+  if (xg && !yg) {
+    s.ghost = true;
+    if (i < 2) {
+      const int bc = g.bc[0];
+      if (bc == 1) {
+        s.i = 3 - i;
+        s.flip = 2;
+      } else if (bc == 2) {
+        s.i = 2;
+      } else {
+        s.i = g.nx + i;
+      }
+    } else {
+      const int bc = g.bc[1];
+      if (bc == 1) {
+        s.i = 2 * g.nx + 3 - i;
+        s.flip = 2;
+      } else if (bc == 2) {
+        s.i = g.nx + 1;
+      } else {
+        s.i = i - g.nx;
+      }
+    }
+  } else if (yg && !xg) {
+    s.ghost = true;
+    if (j < 2) {
+      const int bc = g.bc[2];
+      if (bc == 1) {
+        s.j = 3 - j;
+        s.flip = 3;
+      } else if (bc == 2) {
+        s.j = 2;
+      } else {
+        s.j = g.ny + j;
+      }
+    } else {
+      const int bc = g.bc[3];
+      if (bc == 1) {
+        s.j = 2 * g.ny + 3 - j;
+        s.flip = 3;
+      } else if (bc == 2) {
+        s.j = g.ny + 1;
+      } else {
+        s.j = j - g.ny;
+      }
+    }
+  }
+  return s;
+}
+
+__device__ __forceinline__ int64_t at(const Grid& g, int i, int j, int v) {
+  return static_cast<int64_t>(v) * g.plane + static_cast<int64_t>(j) * g.pitch + i;
+}
+
+// ---------------------------------------------------------------------------
+// cross-lane helpers (wave64)
+template <class T>
+__device__ __forceinline__ T from_left(T x) {  // value of lane-1
+  return __shfl_up(x, 1, 64);
+}
+template <class T>
+__device__ __forceinline__ T from_right(T x) {  // value of lane+1
+  return __shfl_down(x, 1, 64);
+}
+template <class T>
+__device__ __forceinline__ T wave_max(T x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const T y = __shfl_xor(x, o, 64);
+    x = x > y ? x : y;
+  }
+  return x;
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(double* dst, double v) {
+  // IEEE order of non-negative doubles equals the order of their bit patterns
+  atomicMax(reinterpret_cast<unsigned long long*>(dst), static_cast<unsigned long long>(__double_as_longlong(v)));
+}
+
+// XCD-aware block order: hardware hands consecutive workgroups to the 8 XCDs
+// in turn; this gives each XCD a contiguous range of logical blocks so that
+// neighbouring strips/segments (which share halo rows and columns) meet in the
+// same L2.
+__device__ __forceinline__ int xcd_logical_block(int b, int nb) {
+  constexpr int kXcd = 8;
+  const int q = nb / kXcd, r = nb % kXcd;
+  const int xcd = b % kXcd, k = b / kXcd;
+  return (xcd < r) ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
+}
+
+
+// cross-lane shuffles of a dual number move both parts
+template <>
+__device__ __forceinline__ Dual from_left<Dual>(Dual x) {
+  return Dual(__shfl_up(x.v, 1, 64), __shfl_up(x.d, 1, 64));
+}
+template <>
+__device__ __forceinline__ Dual from_right<Dual>(Dual x) {
+  return Dual(__shfl_down(x.v, 1, 64), __shfl_down(x.d, 1, 64));
+}
+
+}  // namespace e2d

@@ -54,9 +54,23 @@ hipError_t launch_sums(const e2d_params& p, const DevBuffers& b, int buf, double
 // conversion between the float64 host layout (isize*jsize*4, Fortran order)
 // and the pitched device state, through a device staging array of doubles
 hipError_t launch_pack(const e2d_params& p, const DevBuffers& b, int buf, const double* d_src, hipStream_t s);
+// host-layout FP64 staging -> FP64 field in the device layout
+hipError_t launch_pack_field(const e2d_params& p, const DevBuffers& b, const double* d_src, double* d_field,
+                             hipStream_t s);
 hipError_t launch_unpack(const e2d_params& p, const DevBuffers& b, const void* d_state, int elem_kind,
                          double* d_dst, hipStream_t s);
 hipError_t launch_pointwise(const e2d_params& p, int func, int n, const double* d_x, double* d_y,
                             hipStream_t s);
+// tangent-linear step at the state of buffer buf (FP64): v_out = J v_in, dt frozen;
+// dual_in/dual_out are two scratch state arrays of 16-byte dual numbers
+hipError_t launch_tangent_step(const e2d_params& p, const DevBuffers& b, int buf, double dt, const double* v_in,
+                               double* v_out, void* dual_in, void* dual_out, hipStream_t s);
+// adjoint step at the state of buffer buf (FP64): lam_out = J^T lam_in, dt frozen
+struct AdjointScratch {
+  double *q = nullptr, *xminb = nullptr, *xmaxb = nullptr, *yminb = nullptr, *ymaxb = nullptr;
+  double *qb = nullptr, *dqxb = nullptr, *dqyb = nullptr, *ub = nullptr;
+};
+hipError_t launch_adjoint_step(const e2d_params& p, const DevBuffers& b, int buf, double dt,
+                               const double* lam_in, double* lam_out, const AdjointScratch& a, hipStream_t s);
 
 }  // namespace e2d

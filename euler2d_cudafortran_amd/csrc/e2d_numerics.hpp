@@ -12,7 +12,15 @@
 // (serial_ref/m_constants.f90:11-14).
 #pragma once
 
+// Host and device: the kernels use these routines; the adjoint's checks also
+// compile them for the CPU.
+#if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
+#define E2D_HDI __host__ __device__ __forceinline__
+#else
+#include <cmath>
+#define E2D_HDI inline __attribute__((always_inline))
+#endif
 
 namespace e2d {
 
@@ -31,11 +39,12 @@ struct Hydro {
 // operations (bitwise parity).  FastMath (float only) uses the native
 // reciprocal and square root (about 1 ulp), for the mixed-precision variant.
 struct ExactMath {
-  static __device__ __forceinline__ double div(double a, double b) { return a / b; }
-  static __device__ __forceinline__ float div(float a, float b) { return a / b; }
-  static __device__ __forceinline__ double sqrt(double a) { return __builtin_sqrt(a); }
-  static __device__ __forceinline__ float sqrt(float a) { return __builtin_sqrtf(a); }
+  static E2D_HDI double div(double a, double b) { return a / b; }
+  static E2D_HDI float div(float a, float b) { return a / b; }
+  static E2D_HDI double sqrt(double a) { return __builtin_sqrt(a); }
+  static E2D_HDI float sqrt(float a) { return __builtin_sqrtf(a); }
 };
+#if defined(__HIPCC__)
 struct FastMath {
   static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
   static __device__ __forceinline__ float sqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
@@ -78,24 +87,25 @@ struct RangedMath {
     return x == 0.0 ? x : s;
   }
 };
+#endif
 
 // Fortran max/min as the reference compiler lowers them (first operand wins
 // only when strictly greater / smaller; this fixes the sign of zero results).
 template <class T>
-__device__ __forceinline__ T fmax2(T a, T b) { return a > b ? a : b; }
+E2D_HDI T fmax2(T a, T b) { return a > b ? a : b; }
 template <class T>
-__device__ __forceinline__ T fmin2(T a, T b) { return a < b ? a : b; }
+E2D_HDI T fmin2(T a, T b) { return a < b ? a : b; }
 template <class T>
-__device__ __forceinline__ T fabs_(T a) { return __builtin_fabs(a); }
+E2D_HDI T fabs_(T a) { return __builtin_fabs(a); }
 template <>
-__device__ __forceinline__ float fabs_<float>(float a) { return __builtin_fabsf(a); }
+E2D_HDI float fabs_<float>(float a) { return __builtin_fabsf(a); }
 // s * x for s = +-1, exactly
 template <class T>
-__device__ __forceinline__ T signed_by(bool positive, T x) { return positive ? x : -x; }
+E2D_HDI T signed_by(bool positive, T x) { return positive ? x : -x; }
 
 // Conservative -> primitive (computePrimitives + eos, m_utils.f90:35-64, 15-29).
 template <class T, class M>
-__device__ __forceinline__ void cons_to_prim(const Hydro<T>& h, const T u[4], T q[4]) {
+E2D_HDI void cons_to_prim(const Hydro<T>& h, const T u[4], T q[4]) {
   const T rho = fmax2(u[0], h.smallr);
   const T vx = M::div(u[2], rho);
   const T vy = M::div(u[3], rho);
@@ -109,14 +119,14 @@ __device__ __forceinline__ void cons_to_prim(const Hydro<T>& h, const T u[4], T 
 
 // Sound speed of a primitive state (second half of eos).
 template <class T, class M>
-__device__ __forceinline__ T sound_speed(const Hydro<T>& h, const T q[4]) {
+E2D_HDI T sound_speed(const Hydro<T>& h, const T q[4]) {
   return M::sqrt(M::div(h.gamma0 * q[1], q[0]));
 }
 
 // Local CFL rate (c+|u|)/dx + (c+|v|)/dy of a conservative state
 // (compute_dt body, serial_ref/m_run.f90:278-281).
 template <class T, class M>
-__device__ __forceinline__ T cfl_rate(const Hydro<T>& h, const T u[4]) {
+E2D_HDI T cfl_rate(const Hydro<T>& h, const T u[4]) {
   T q[4];
   cons_to_prim<T, M>(h, u, q);
   const T c = sound_speed<T, M>(h, q);
@@ -128,7 +138,7 @@ __device__ __forceinline__ T cfl_rate(const Hydro<T>& h, const T u[4]) {
 // Limited slope of one variable along one direction (slope_unsplit_hydro_2d,
 // m_utils.f90:298-312): minmod for slope_type 1, MC for 2.
 template <class T>
-__device__ __forceinline__ T limited_slope(T st, T qm, T q0, T qp) {
+E2D_HDI T limited_slope(T st, T qm, T q0, T qp) {
   const T dl = st * (q0 - qm);
   const T dr = st * (qp - q0);
   const T dc = T(0.5) * (qp - qm);
@@ -140,7 +150,7 @@ __device__ __forceinline__ T limited_slope(T st, T qm, T q0, T qp) {
 // Slopes of a cell in both directions.  slope_type 0 and values other than
 // 1/2 give zero slopes (first order), m_utils.f90:289-337.
 template <class T>
-__device__ __forceinline__ void cell_slopes(const Hydro<T>& h, const T q[4], const T qxp[4],
+E2D_HDI void cell_slopes(const Hydro<T>& h, const T q[4], const T qxp[4],
                                             const T qxm[4], const T qyp[4], const T qym[4],
                                             T dqx[4], T dqy[4]) {
   const bool limited = (h.slope_type == 1 || h.slope_type == 2);
@@ -157,7 +167,7 @@ __device__ __forceinline__ void cell_slopes(const Hydro<T>& h, const T q[4], con
 // Hancock predictor source terms of a cell (trace_unsplit_2d_along_dir,
 // m_utils.f90:213-216): s = (sr0, sp0, su0, sv0).
 template <class T, class M>
-__device__ __forceinline__ void trace_sources(const Hydro<T>& h, const T q[4], const T dqx[4],
+E2D_HDI void trace_sources(const Hydro<T>& h, const T q[4], const T dqx[4],
                                               const T dqy[4], T s[4]) {
   const T r = q[0], p = q[1], u = q[2], v = q[3];
   const T div_vel = dqx[2] + dqy[3];
@@ -171,7 +181,7 @@ __device__ __forceinline__ void trace_sources(const Hydro<T>& h, const T q[4], c
 // floored (m_utils.f90:218-252).  side = +1 for the XMAX/YMAX face, -1 for
 // XMIN/YMIN; dq and dtd are those of the face's direction.
 template <class T>
-__device__ __forceinline__ void face_state(const Hydro<T>& h, const T q[4], const T dq[4],
+E2D_HDI void face_state(const Hydro<T>& h, const T q[4], const T dq[4],
                                            const T s[4], T dtd, bool plus_side, T f[4]) {
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
@@ -184,7 +194,7 @@ __device__ __forceinline__ void face_state(const Hydro<T>& h, const T q[4], cons
 
 // Flux of a Godunov state (cmpflx, m_utils.f90:498-526).
 template <class T>
-__device__ __forceinline__ void godunov_flux(const Hydro<T>& h, const T g[4], T f[4]) {
+E2D_HDI void godunov_flux(const Hydro<T>& h, const T g[4], T f[4]) {
   const T mass = g[0] * g[2];
   f[0] = mass;
   f[2] = mass * g[2] + g[1];
@@ -198,7 +208,7 @@ __device__ __forceinline__ void godunov_flux(const Hydro<T>& h, const T g[4], T 
 // Newton iterations on p*, star state, sampling at x/t = 0, transverse
 // velocity upwinded by the contact.  Writes the Godunov state and its flux.
 template <class T, class M>
-__device__ __forceinline__ void riemann_two_shock(const Hydro<T>& h, const T ql[4], const T qr[4],
+E2D_HDI void riemann_two_shock(const Hydro<T>& h, const T ql[4], const T qr[4],
                                                   T g[4], T f[4]) {
   const T rl = fmax2(ql[0], h.smallr);
   const T ul = ql[2];
@@ -271,7 +281,7 @@ __device__ __forceinline__ void riemann_two_shock(const Hydro<T>& h, const T ql[
 // HLLC solver (riemann_2d_hllc, m_utils.f90:534-644).  The reference never
 // selects it in the step; here riemann_solver == 2 does.
 template <class T, class M>
-__device__ __forceinline__ void riemann_hllc(const Hydro<T>& h, const T ql[4], const T qr[4],
+E2D_HDI void riemann_hllc(const Hydro<T>& h, const T ql[4], const T qr[4],
                                              T g[4], T f[4]) {
   const T H = T(0.5);
   const T rl = fmax2(ql[0], h.smallr);
@@ -322,7 +332,7 @@ __device__ __forceinline__ void riemann_hllc(const Hydro<T>& h, const T ql[4], c
 // HLLC divides by wave-speed differences that can vanish, so it always keeps
 // IEEE division (the reference's inf/NaN behaviour included).
 template <class T, class M, int SOLVER>
-__device__ __forceinline__ void riemann(const Hydro<T>& h, const T ql[4], const T qr[4], T g[4], T f[4]) {
+E2D_HDI void riemann(const Hydro<T>& h, const T ql[4], const T qr[4], T g[4], T f[4]) {
   if constexpr (SOLVER == 2)
     riemann_hllc<T, ExactMath>(h, ql, qr, g, f);
   else
@@ -332,7 +342,7 @@ __device__ __forceinline__ void riemann(const Hydro<T>& h, const T ql[4], const 
 // Flux across a y-face: the x-solver with the velocity components exchanged
 // on input and output (serial_ref/m_run.f90:463-470).
 template <class T, class M, int SOLVER>
-__device__ __forceinline__ void riemann_y(const Hydro<T>& h, const T ql[4], const T qr[4], T f[4]) {
+E2D_HDI void riemann_y(const Hydro<T>& h, const T ql[4], const T qr[4], T f[4]) {
   const T l[4] = {ql[0], ql[1], ql[3], ql[2]};
   const T r[4] = {qr[0], qr[1], qr[3], qr[2]};
   T g[4], fs[4];

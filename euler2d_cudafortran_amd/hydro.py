@@ -81,6 +81,19 @@ _SIGS = {
                                          ctypes.POINTER(ctypes.c_int64)]),
     "e2d_pointwise": (ctypes.c_int, [ctypes.POINTER(_Params), ctypes.c_int, ctypes.c_int,
                                      ctypes.c_void_p, ctypes.c_void_p]),
+    "e2d_tangent_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
+                                        ctypes.c_void_p]),
+    "e2d_adjoint_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
+                                        ctypes.c_void_p]),
+    "e2d_adjoint_rollout": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                           ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.POINTER(ctypes.c_int64)]),
+    "e2d_field_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "e2d_field_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "e2d_field_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "e2d_field_download": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "e2d_copy_to_buffer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "e2d_copy_from_buffer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
 }
 
 
@@ -349,6 +362,46 @@ class HydroRun:
         pitch = ctypes.c_int64()
         _check(self.lib.e2d_device_buffer(self.ctx, buf, ctypes.byref(p), ctypes.byref(pitch)))
         return p.value, pitch.value
+
+    # -- tangent / adjoint (FP64, two-shock solver) ------------------------
+    def field_alloc(self) -> int:
+        """Zero FP64 vector field in the device layout; returns its device address."""
+        p = ctypes.c_void_p()
+        _check(self.lib.e2d_field_alloc(self.ctx, ctypes.byref(p)))
+        return p.value
+
+    def field_free(self, d: int):
+        _check(self.lib.e2d_field_free(self.ctx, d))
+
+    def field_upload(self, d: int, h: np.ndarray):
+        a = np.ascontiguousarray(np.asfortranarray(h, dtype=np.float64).reshape(-1, order="F"))
+        _check(self.lib.e2d_field_upload(self.ctx, d, a.ctypes.data))
+
+    def field_download(self, d: int) -> np.ndarray:
+        out = np.empty(self.shape[0] * self.shape[1] * 4, dtype=np.float64)
+        _check(self.lib.e2d_field_download(self.ctx, d, out.ctypes.data))
+        return out.reshape(self.shape, order="F")
+
+    def tangent_step(self, buf: int, dt: float, d_v_in: int, d_v_out: int):
+        _check(self.lib.e2d_tangent_step(self.ctx, buf, dt, d_v_in, d_v_out))
+
+    def adjoint_step(self, buf: int, dt: float, d_lam_in: int, d_lam_out: int):
+        _check(self.lib.e2d_adjoint_step(self.ctx, buf, dt, d_lam_in, d_lam_out))
+
+    def adjoint_rollout(self, n_step0: int, dts, nsnap: int, d_lam_n: int, d_lam_0: int) -> int:
+        """lam_0 = (dU_n/dU_0)^T lam_n over len(dts) frozen-dt steps from the state in
+        buffer n_step0 % 2; returns the number of forward steps recomputed."""
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        nf = ctypes.c_int64()
+        _check(self.lib.e2d_adjoint_rollout(self.ctx, n_step0, d.size, d.ctypes.data, nsnap, d_lam_n, d_lam_0,
+                                            ctypes.byref(nf)))
+        return nf.value
+
+    def copy_to_buffer(self, buf: int, d_src: int):
+        _check(self.lib.e2d_copy_to_buffer(self.ctx, buf, d_src))
+
+    def copy_from_buffer(self, buf: int, d_dst: int):
+        _check(self.lib.e2d_copy_from_buffer(self.ctx, buf, d_dst))
 
     def current(self) -> int:
         """Buffer holding the latest state (u for even nStep, u2 for odd)."""

@@ -193,6 +193,44 @@ int e2d_get_timers(e2d_ctx *ctx, double ms[E2D_NUM_TIMERS], int64_t count[E2D_NU
  * precision's storage type (double for FP64, float for FP32/MIXED). */
 int e2d_device_buffer(e2d_ctx *ctx, int buf, void **dptr, int64_t *pitch);
 
+/*
+ * Tangent-linear and adjoint of the step (no counterpart in the reference).
+ * FP64 and the two-shock solver only.  The step is linearised as executed
+ * (floors, limiter branches, sampling branches and Newton iteration counts of
+ * the forward pass, |x|' = +1 at 0) with dt frozen.  Vector fields are FP64
+ * device arrays in the state's device layout (element (i,j,v) at
+ * [(v*jsize + j)*pitch + i], see e2d_device_buffer); only interior entries
+ * are read and written.
+ */
+
+/* v_out = J v_in for the step whose input state is buffer buf (the state is
+ * not modified apart from its ghost cells, which are filled). */
+int e2d_tangent_step(e2d_ctx *ctx, int buf, double dt, const double *d_v_in, double *d_v_out);
+
+/* lam_out = J^T lam_in for the same step; lam_out may alias lam_in. */
+int e2d_adjoint_step(e2d_ctx *ctx, int buf, double dt, const double *d_lam_in, double *d_lam_out);
+
+/* Gradient of an nsteps rollout from the state in buffer nStep0 % 2 with the
+ * frozen dt sequence h_dts[0..nsteps-1]: lam_0 = (dU_n / dU_0)^T lam_n.
+ * Binomial checkpointing (recursive treeverse with Griewank's split) keeps at
+ * most nsnap state snapshots in device memory; *n_forward returns the number
+ * of forward steps recomputed.  On return the starting state is back in its
+ * buffer. */
+int e2d_adjoint_rollout(e2d_ctx *ctx, int nStep0, int nsteps, const double *h_dts, int nsnap,
+                        const double *d_lam_n, double *d_lam_0, int64_t *n_forward);
+
+/* FP64 device vector fields in the state's layout (zero-filled on allocation),
+ * and their transfer from/to host arrays in the layout of e2d_upload. */
+int e2d_field_alloc(e2d_ctx *ctx, double **d_field);
+int e2d_field_free(e2d_ctx *ctx, double *d_field);
+int e2d_field_upload(e2d_ctx *ctx, double *d_field, const double *h);
+int e2d_field_download(e2d_ctx *ctx, const double *d_field, double *h);
+
+/* Device-to-device copy of a whole state buffer (device layout, storage
+ * type) from/to caller memory, e.g. torch tensors of shape (4, jsize, pitch). */
+int e2d_copy_to_buffer(e2d_ctx *ctx, int buf, const void *d_src);
+int e2d_copy_from_buffer(e2d_ctx *ctx, int buf, void *d_dst);
+
 /* Pointwise device numerics over a batch, for known-answer tests of the
  * device routines against the reference's (serial_ref/m_utils.f90).
  * x holds n items of the input width of func, y receives n items of its

@@ -1,0 +1,219 @@
+"""Tangent-linear and adjoint of the step on the GPU (FP64, frozen dt).
+
+* dot-product identity <J v, w> = <v, J^T w> (tangent = dual numbers through
+  the forward kernel, adjoint = hand-written reverse kernels), on
+  discontinuous states, for all boundary and slope types;
+* tangent against 4-point finite differences of the forward step;
+* the checkpointed rollout adjoint: independent of the number of snapshots
+  (bit for bit), consistent with the chained tangent, and against finite
+  differences of a rollout objective.
+"""
+import numpy as np
+import pytest
+
+import euler2d_cudafortran_amd as e2d
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(nx, ny=None, **kw):
+    p = e2d.HydroParams(nx=nx, ny=nx if ny is None else ny)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def _interior(nx, ny):
+    m = np.zeros((nx + 4, ny + 4, 4), dtype=bool)
+    m[2:nx + 2, 2:ny + 2, :] = True
+    return m
+
+
+def _smooth_field(nx, ny, rng, modes=4):
+    x = (np.arange(nx + 4) - 1.5) / nx
+    y = (np.arange(ny + 4) - 1.5) / ny
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    f = np.zeros_like(X)
+    for _ in range(modes):
+        kx, ky = rng.integers(1, 4, 2)
+        f += rng.normal() * np.sin(2 * np.pi * (kx * X + rng.uniform())) * np.cos(2 * np.pi * (ky * Y + rng.uniform()))
+    return f / modes
+
+
+def _smooth_state(nx, ny, seed, gamma=1.4):
+    rng = np.random.default_rng(seed)
+    rho = 1.0 + 0.3 * _smooth_field(nx, ny, rng)
+    u = 0.2 * _smooth_field(nx, ny, rng)
+    v = 0.2 * _smooth_field(nx, ny, rng)
+    p = 1.0 + 0.3 * _smooth_field(nx, ny, rng)
+    U = np.stack([rho, p / (gamma - 1) + 0.5 * rho * (u * u + v * v), rho * u, rho * v], axis=-1)
+    return U
+
+
+def _evolved_diag(run, steps):
+    run.init("diag")
+    run.make_boundaries(0)
+    run.run_steps(steps)
+    return run.download(run.current())
+
+
+@pytest.mark.parametrize("bc,st", [((1, 1, 1, 1), 1), ((2, 2, 2, 2), 2), ((3, 3, 3, 3), 0), ((1, 2, 3, 1), 1)])
+def test_dot_product_single_step(gpu_lib, bc, st):
+    nx, ny = 96, 80
+    p = _params(nx, ny, boundary_type_xmin=bc[0], boundary_type_xmax=bc[1], boundary_type_ymin=bc[2],
+                boundary_type_ymax=bc[3], slope_type=st)
+    rng = np.random.default_rng(7)
+    mask = _interior(nx, ny)
+    with e2d.HydroRun(p) as run:
+        U = _evolved_diag(run, 30)
+        run.upload(0, U)
+        dt = run.compute_dt(0)
+        v = np.where(mask, rng.normal(size=mask.shape), 0.0)
+        w = np.where(mask, rng.normal(size=mask.shape), 0.0)
+        dv, djv, dw, djw = (run.field_alloc() for _ in range(4))
+        run.field_upload(dv, v)
+        run.field_upload(dw, w)
+        run.tangent_step(0, dt, dv, djv)
+        run.upload(0, U)
+        run.adjoint_step(0, dt, dw, djw)
+        jv, jtw = run.field_download(djv), run.field_download(djw)
+        for d in (dv, djv, dw, djw):
+            run.field_free(d)
+    lhs = float(np.sum(jv[mask] * w[mask]))
+    rhs = float(np.sum(v[mask] * jtw[mask]))
+    rel = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    print(f"dot test bc={bc} slope_type={st}: <Jv,w>={lhs:.15e} <v,J^T w>={rhs:.15e} rel={rel:.2e}")
+    assert rel <= 1e-10
+
+
+def _forward(run, U, dt):
+    run.upload(0, U)
+    run.godunov_unsplit(0, dt)
+    return run.download(1)
+
+
+@pytest.mark.parametrize("st,tol", [(0, 1e-8), (1, 1e-8)])
+def test_tangent_vs_finite_differences(gpu_lib, st, tol):
+    nx, ny = 64, 48
+    p = _params(nx, ny, slope_type=st, boundary_type_xmin=3, boundary_type_xmax=3, boundary_type_ymin=3,
+                boundary_type_ymax=3)
+    U = _smooth_state(nx, ny, 11)
+    mask = _interior(nx, ny)
+    rng = np.random.default_rng(12)
+    V = np.stack([_smooth_field(nx, ny, rng) for _ in range(4)], axis=-1)
+    V = np.where(mask, V, 0.0)
+    with e2d.HydroRun(p) as run:
+        run.upload(0, U)
+        dt = run.compute_dt(0)
+        dv, djv = run.field_alloc(), run.field_alloc()
+        run.field_upload(dv, V)
+        run.upload(0, U)
+        run.tangent_step(0, dt, dv, djv)
+        jv = run.field_download(djv)
+        # small enough that no branch of the step (limiter, contact direction,
+        # Newton count) switches between the perturbed runs
+        eps = 1e-6
+        f = {s: _forward(run, U + s * eps * V, dt) for s in (-2, -1, 1, 2)}
+    fd = (8 * (f[1] - f[-1]) - (f[2] - f[-2])) / (12 * eps)
+    err = np.linalg.norm((fd - jv)[mask]) / np.linalg.norm(jv[mask])
+    print(f"tangent vs FD, slope_type={st}: relative L2 error {err:.2e}")
+    assert err <= tol
+
+
+def _record_forward(run, U0, nsteps):
+    run.upload(0, U0)
+    run.nStep = 0
+    dts = []
+    for n in range(nsteps):
+        dt = run.compute_dt(n % 2)
+        dts.append(dt)
+        run.godunov_unsplit(n, dt)
+    return np.array(dts)
+
+
+def test_rollout_snapshots_and_tangent_chain(gpu_lib):
+    nx, ny, nsteps = 64, 64, 24
+    mask = _interior(nx, ny)
+    rng = np.random.default_rng(21)
+    with e2d.HydroRun(_params(nx, ny)) as run:
+        U0 = _evolved_diag(run, 10)
+        run.init("diag")  # corners of both buffers as the run starts
+        dts = _record_forward(run, U0, nsteps)
+        lam_n = np.where(mask, rng.normal(size=mask.shape), 0.0)
+        d_n, d_0 = run.field_alloc(), run.field_alloc()
+        run.field_upload(d_n, lam_n)
+        results = {}
+        for nsnap in (1, 3, 6, nsteps):
+            run.upload(0, U0)
+            nf = run.adjoint_rollout(0, dts, nsnap, d_n, d_0)
+            results[nsnap] = (run.field_download(d_0), nf)
+            print(f"nsnap={nsnap}: {nf} forward steps recomputed")
+        # tangent chained through the same states
+        v0 = np.where(mask, rng.normal(size=mask.shape), 0.0)
+        dv, dw = run.field_alloc(), run.field_alloc()
+        run.field_upload(dv, v0)
+        run.upload(0, U0)
+        for n in range(nsteps):
+            state = run.download(n % 2)
+            run.tangent_step(n % 2, dts[n], dv, dw)
+            run.upload(n % 2, state)
+            run.godunov_unsplit(n, dts[n])
+            dv, dw = dw, dv
+        vn = run.field_download(dv)
+    ref = results[nsteps][0]
+    for nsnap, (lam0, _) in results.items():
+        assert np.array_equal(lam0, ref), f"nsnap={nsnap} differs"
+    assert results[1][1] > results[3][1] > results[nsteps][1]
+    lhs = float(np.sum(vn[mask] * lam_n[mask]))
+    rhs = float(np.sum(v0[mask] * ref[mask]))
+    rel = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    print(f"rollout dot test: {lhs:.15e} vs {rhs:.15e}, rel {rel:.2e}")
+    assert rel <= 1e-10
+
+
+def test_rollout_gradient_vs_finite_differences(gpu_lib):
+    """J = sum of rho over cells with centres in [0.6,0.8]^2 after the rollout;
+    gradient with respect to the initial rho and p (E = p/(gamma-1) + kinetic),
+    smooth initial state; 4-point finite differences with the dt sequence
+    frozen at the unperturbed run's."""
+    nx, ny, nsteps, gamma = 64, 64, 40, 1.4
+    mask = _interior(nx, ny)
+    xc = (np.arange(nx + 4) - 2 + 0.5) / nx
+    box = np.zeros_like(mask)
+    sel = (xc >= 0.6) & (xc <= 0.8)
+    box[np.ix_(sel, sel, [0])] = True
+    U0 = _smooth_state(nx, ny, 31)
+    rng = np.random.default_rng(32)
+    drho = _smooth_field(nx, ny, rng)
+    dp = _smooth_field(nx, ny, rng)
+    dU = np.zeros_like(U0)
+    rho, u, v = U0[..., 0], U0[..., 2] / U0[..., 0], U0[..., 3] / U0[..., 0]
+    dU[..., 0] = drho
+    dU[..., 1] = dp / (gamma - 1) + 0.5 * drho * (u * u + v * v)
+    dU[..., 2] = drho * u
+    dU[..., 3] = drho * v
+    dU = np.where(mask, dU, 0.0)
+    with e2d.HydroRun(_params(nx, ny, boundary_type_xmin=3, boundary_type_xmax=3, boundary_type_ymin=3,
+                              boundary_type_ymax=3)) as run:
+        run.init("diag")
+        dts = _record_forward(run, U0, nsteps)
+
+        def objective(Ustart):
+            run.upload(0, Ustart)
+            for n in range(nsteps):
+                run.godunov_unsplit(n, dts[n])
+            return float(run.download(nsteps % 2)[box].sum())
+
+        lam_n = np.where(box, 1.0, 0.0)
+        d_n, d_0 = run.field_alloc(), run.field_alloc()
+        run.field_upload(d_n, lam_n)
+        run.upload(0, U0)
+        run.adjoint_rollout(0, dts, 5, d_n, d_0)
+        lam0 = run.field_download(d_0)
+        eps = 1e-6
+        f = {s: objective(U0 + s * eps * dU) for s in (-2, -1, 1, 2)}
+    fd = (8 * (f[1] - f[-1]) - (f[2] - f[-2])) / (12 * eps)
+    ad = float(np.sum(lam0[mask] * dU[mask]))
+    rel = abs(fd - ad) / abs(ad)
+    print(f"rollout gradient: adjoint {ad:.15e}, finite difference {fd:.15e}, rel {rel:.2e}")
+    assert rel <= 1e-8

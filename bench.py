@@ -137,6 +137,59 @@ def precision_error(e2d, nx, steps):
     return out
 
 
+def adjoint_metrics(e2d):
+    """Adjoint cost: one adjoint step against one forward step at 2048^2 FP64
+    (kernel time from hipEvents), and whole-gradient rollouts (forward pass with
+    the dt sequence recorded, then the binomial-checkpointed adjoint) for
+    512^2 x 400 steps with 20 snapshots and 2048^2 x 2000 steps with 45."""
+    out = {}
+    run = make_run(e2d, 2048, 0, e2d.PREC_FP64)
+    run.run_steps(100)
+    run.synchronize()
+    buf, dt = run.current(), run.get_dt()
+    a, b = run.field_alloc(), run.field_alloc()
+    run.enable_timers(True)
+    run.get_timers()
+    for _ in range(20):
+        run.godunov_unsplit(run.nStep, dt)
+    fwd = run.get_timers()["godunov"]
+    for _ in range(20):
+        run.adjoint_step(buf, dt, a, b)
+    adj = run.get_timers()["godunov"]
+    run.close()
+    f_ms, a_ms = fwd[0] / fwd[1], adj[0] / adj[1]
+    out["step_2048_fp64"] = {"forward_kernel_ms": round(f_ms, 4), "adjoint_kernels_ms": round(a_ms, 4),
+                             "adjoint_over_forward": round(a_ms / f_ms, 3)}
+    for name, (nx, nsteps, nsnap) in {"rollout_512_400steps_20snap": (512, 400, 20),
+                                      "rollout_2048_2000steps_45snap": (2048, 2000, 45)}.items():
+        run = make_run(e2d, nx, 0, e2d.PREC_FP64)
+        u0 = run.download(0)
+        dts = []
+        run.synchronize()
+        t0 = time.perf_counter()
+        for n in range(nsteps):
+            d = run.compute_dt(n % 2)
+            dts.append(d)
+            run.godunov_unsplit(n, d)
+        run.synchronize()
+        t_fwd = time.perf_counter() - t0
+        lam_n, lam_0 = run.field_alloc(), run.field_alloc()
+        mask = np.zeros((nx + 4, nx + 4, 4))
+        xc = (np.arange(nx + 4) - 2 + 0.5) / nx
+        sel = (xc >= 0.6) & (xc <= 0.8)
+        mask[np.ix_(sel, sel, [0])] = 1.0  # J = sum of rho over [0.6,0.8]^2
+        run.field_upload(lam_n, mask)
+        run.upload(0, u0)
+        run.synchronize()
+        t0 = time.perf_counter()
+        nf = run.adjoint_rollout(0, dts, nsnap, lam_n, lam_0)
+        t_adj = time.perf_counter() - t0
+        run.close()
+        out[name] = {"forward_s": round(t_fwd, 3), "gradient_s": round(t_adj, 3),
+                     "gradient_over_forward": round(t_adj / t_fwd, 3), "forward_steps_recomputed": nf}
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,13 +265,18 @@ def main(argv=None):
         _log("[bench] precision error, bumps problem 4096^2, 500 steps")
         variants["precision_error_bumps_4096_500steps_vs_fp64"] = precision_error(e2d, 4096, 500)
 
+    adjoint = None
+    if not args.no_variants:
+        _log("[bench] adjoint cost")
+        adjoint = adjoint_metrics(e2d)
+
     cpu = None
     if not args.no_cpu_baseline:
         _log(f"[bench] cpu baseline {nx}x{nx}, {args.cpu_steps} steps")
         cpu = cpu_baseline(nx, args.cpu_steps)
 
     line = {
-        "metric": "Mcell-updates/s at 2048^2 FP64 (fused step), %HBM peak",
+        "metric": "Mcell-updates/s at 2048^2 FP64 vs FP32 (%HBM peak); adjoint/forward cost ratio",
         "value": round(value, 2),
         "unit": "Mcell-updates/s",
         "n_gpus": 1,
@@ -238,11 +296,14 @@ def main(argv=None):
                    "dt_after_run": dt_final},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "limiter": ("FP64 VALU issue, not HBM (profiles/k_step_fused_fp64_2048_valu.txt)"
+                                 if (args.precision, args.impl) == ("fp64", 0) else None),
                      "kernel": "k_step_fused" if args.impl == 0 else "materialised step (5 kernels)",
                      "kernel_avg_ms": round(kernel_ms, 5),
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "variants": variants,
+        "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -1,0 +1,75 @@
+"""torch.autograd harness over the adjoint C-ABI (gradient checks, inverse problems).
+
+`GodunovRollout.apply(u0, run, dts, nsnap)` maps the initial state u0 (FP64
+CUDA tensor of shape (4, jsize, pitch): the library's device layout, see
+`state_tensor`) to the state after len(dts) steps with the frozen dt
+sequence dts.  Its backward is e2d_adjoint_rollout (binomial checkpointing
+with nsnap snapshots).  Torch only allocates and moves memory here; every
+step and every adjoint step runs in libeuler2d_hip.  Ghost and corner entries
+of the gradient are zero (ghosts follow the interior through the boundary
+conditions; corners are treated as constants).
+"""
+from __future__ import annotations
+
+import torch
+
+from .hydro import PREC_FP64, HydroRun
+
+
+def state_tensor(run: HydroRun, buf: int) -> torch.Tensor:
+    """Copy of state buffer `buf` as a (4, jsize, pitch) float64 CUDA tensor."""
+    if run.params.precision != PREC_FP64:
+        raise ValueError("the autograd harness needs precision 0 (fp64)")
+    _, pitch = run.device_buffer(buf)
+    t = torch.empty((4, run.params.jsize, pitch), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    run.copy_from_buffer(buf, t.data_ptr())
+    return t
+
+
+def interior_mask(run: HydroRun, like: torch.Tensor) -> torch.Tensor:
+    m = torch.zeros_like(like, dtype=torch.bool)
+    m[:, 2:run.params.ny + 2, 2:run.params.nx + 2] = True
+    return m
+
+
+def record_dts(run: HydroRun, u0: torch.Tensor, nsteps: int) -> list[float]:
+    """Forward run from u0 with the reference's dt rule (compute_dt each
+    step, running maximum reset first); returns the dt sequence."""
+    run.invdt = 0.0
+    torch.cuda.synchronize()
+    run.copy_to_buffer(0, u0.data_ptr())
+    dts = []
+    for n in range(nsteps):
+        dt = run.compute_dt(n % 2)
+        dts.append(dt)
+        run.godunov_unsplit(n, dt)
+    run.synchronize()
+    return dts
+
+
+class GodunovRollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u0: torch.Tensor, run: HydroRun, dts, nsnap: int):
+        u0 = u0.contiguous()
+        torch.cuda.synchronize()
+        run.copy_to_buffer(0, u0.data_ptr())
+        for n, dt in enumerate(dts):
+            run.godunov_unsplit(n, float(dt))
+        out = torch.empty_like(u0)
+        run.copy_from_buffer(len(dts) % 2, out.data_ptr())
+        ctx.run, ctx.dts, ctx.nsnap = run, [float(d) for d in dts], int(nsnap)
+        ctx.save_for_backward(u0)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out: torch.Tensor):
+        (u0,) = ctx.saved_tensors
+        run = ctx.run
+        g = grad_out.contiguous()
+        lam0 = torch.zeros_like(g)
+        torch.cuda.synchronize()
+        run.copy_to_buffer(0, u0.data_ptr())
+        run.adjoint_rollout(0, ctx.dts, ctx.nsnap, g.data_ptr(), lam0.data_ptr())
+        lam0 = torch.where(interior_mask(run, lam0), lam0, torch.zeros_like(lam0))
+        return lam0, None, None, None

@@ -254,6 +254,7 @@ def main(argv=None):
     if not args.no_variants:
         for name, (vnx, impl, vprec, vsteps) in {
             "fp64_materialised_2048": (2048, 1, e2d.PREC_FP64, 200),
+            "fp64_flux_store_2048": (2048, 2, e2d.PREC_FP64, 200),
             "fp32_fused_2048": (2048, 0, e2d.PREC_FP32, 500),
             "mixed_fused_2048": (2048, 0, e2d.PREC_MIXED, 500),
             "fp64_fused_4096": (4096, 0, e2d.PREC_FP64, 200),
@@ -298,7 +299,8 @@ def main(argv=None):
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "limiter": ("FP64 VALU issue, not HBM (profiles/k_step_fused_fp64_2048_valu.txt)"
                                  if (args.precision, args.impl) == ("fp64", 0) else None),
-                     "kernel": "k_step_fused" if args.impl == 0 else "materialised step (5 kernels)",
+                     "kernel": {0: "k_step_fused", 1: "materialised step (5 kernels)",
+                                2: "flux-store step (3 kernels)"}[args.impl],
                      "kernel_avg_ms": round(kernel_ms, 5),
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,

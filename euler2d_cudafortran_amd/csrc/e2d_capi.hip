@@ -46,9 +46,10 @@ int validate(const e2d_params* p) {
   if (p->nx < 4 || p->ny < 4) return fail(E2D_ERR_ARG, "nx and ny must be >= 4 (got %d, %d)", p->nx, p->ny);
   if (static_cast<int64_t>(p->nx + 4) * (p->ny + 4) > (int64_t(1) << 31) / 4)
     return fail(E2D_ERR_ARG, "grid %d x %d too large", p->nx, p->ny);
-  if (p->implementation_version != E2D_IMPL_FUSED && p->implementation_version != E2D_IMPL_MATERIALISED)
+  if (p->implementation_version != E2D_IMPL_FUSED && p->implementation_version != E2D_IMPL_MATERIALISED &&
+      p->implementation_version != E2D_IMPL_FLUX_STORE)
     return fail(E2D_ERR_ARG,
-                "The implementation version parameter should be 0 or 1 (got %d); check the namelist, section OTHER",
+                "The implementation version parameter should be 0, 1 or 2 (got %d); check the namelist, section OTHER",
                 p->implementation_version);
   if (p->precision < E2D_PREC_FP64 || p->precision > E2D_PREC_MIXED)
     return fail(E2D_ERR_ARG, "precision must be 0 (fp64), 1 (fp32) or 2 (mixed), got %d", p->precision);
@@ -237,8 +238,10 @@ int e2d_create(const e2d_params* p, e2d_ctx** out) {
             hipMalloc(reinterpret_cast<void**>(&c->b.scal), 4 * sizeof(double)) == hipSuccess &&
             hipMalloc(reinterpret_cast<void**>(&c->b.partial), e2d::kSumBlocks * 4 * sizeof(double)) == hipSuccess &&
             hipMalloc(reinterpret_cast<void**>(&c->d_sums), 4 * sizeof(double)) == hipSuccess;
-  if (ok && p->implementation_version == E2D_IMPL_MATERIALISED) {
-    for (int s = 0; s < e2d::kNumStages && ok; ++s) {
+  if (ok && p->implementation_version != E2D_IMPL_FUSED) {
+    // version 1 keeps every stage; version 2 only q, fx and fy (cuf/m_HydroRun_gpu.cuf:93-105)
+    const int nstages = p->implementation_version == E2D_IMPL_MATERIALISED ? e2d::kNumStages : E2D_STAGE_FY + 1;
+    for (int s = 0; s < nstages && ok; ++s) {
       const size_t bytes = n * e2d::stage_elem_bytes(p->precision, s);
       ok = hipMalloc(&c->b.stage[s], bytes) == hipSuccess &&
            hipMemsetAsync(c->b.stage[s], 0, bytes, c->stream) == hipSuccess;
@@ -322,8 +325,10 @@ int e2d_download_stage(e2d_ctx* c, int stage, double* h) {
   if (int rc = check_ctx(c)) return rc;
   if (!h) return fail(E2D_ERR_ARG, "host array is NULL");
   if (stage < 0 || stage >= e2d::kNumStages) return fail(E2D_ERR_ARG, "unknown stage %d", stage);
-  if (c->p.implementation_version != E2D_IMPL_MATERIALISED)
-    return fail(E2D_ERR_STATE, "stage arrays exist only with implementationVersion 1");
+  if (c->p.implementation_version == E2D_IMPL_FUSED)
+    return fail(E2D_ERR_STATE, "stage arrays exist only with implementationVersion 1 or 2");
+  if (!c->b.stage[stage])
+    return fail(E2D_ERR_STATE, "implementationVersion 2 keeps only the q, fx and fy stages");
   return download_array(c, c->b.stage[stage], static_cast<int>(e2d::stage_elem_bytes(c->p.precision, stage)), h);
 }
 

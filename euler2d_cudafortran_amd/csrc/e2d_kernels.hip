@@ -462,7 +462,8 @@ hipError_t launch_fill(double* dst, int n, double v, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// materialised step (implementationVersion 1)
+// materialised steps (implementationVersion 1: q, slopes, face states and
+// fluxes in HBM; implementationVersion 2: q and fluxes in HBM)
 
 template <class P>
 struct MatArgs {
@@ -588,6 +589,58 @@ __global__ __launch_bounds__(256) void k_flux(const MatArgs<P> a) {
   for (int v = 0; v < 4; ++v) a.fy[at(a.g, i, j, v)] = static_cast<U>(f[v]) * dtdy;
 }
 
+// implementationVersion 2: fluxes straight from q, with the slopes of the
+// cell and of its i-1 and j-1 neighbours evaluated in registers and only fx, fy
+// stored (the structure of computeFluxesAndStore, cuf/m_HydroRun_gpu.cuf:747-884;
+// the arithmetic of computeAndStoreFluxes, serial_ref/m_run.f90:389-480).
+// Faces (i, j) in [2, size-2]^2.
+template <class P>
+__device__ __forceinline__ void cell_faces_from_q(const MatArgs<P>& a, int i, int j, typename P::T q[4],
+                                                  typename P::T dx[4], typename P::T dy[4], typename P::T s[4]) {
+  using T = typename P::T;
+  T xp[4], xm[4], yp[4], ym[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    q[v] = a.q[at(a.g, i, j, v)];
+    xp[v] = a.q[at(a.g, i + 1, j, v)];
+    xm[v] = a.q[at(a.g, i - 1, j, v)];
+    yp[v] = a.q[at(a.g, i, j + 1, v)];
+    ym[v] = a.q[at(a.g, i, j - 1, v)];
+  }
+  cell_slopes(a.hT, q, xp, xm, yp, ym, dx, dy);
+  trace_sources<T, typename P::M>(a.hT, q, dx, dy, s);
+}
+
+template <class P, int SOLVER>
+__global__ __launch_bounds__(256) void k_flux_store(const MatArgs<P> a) {
+  using T = typename P::T;
+  using U = typename P::U;
+  const int i = 2 + blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = 2 + blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (i > a.g.isize - 2 || j > a.g.jsize - 2) return;
+  U dtdxU, dtdyU;
+  step_dt(a, dtdxU, dtdyU);
+  const T dtdx = static_cast<T>(dtdxU), dtdy = static_cast<T>(dtdyU);
+  T q[4], dx[4], dy[4], s[4];
+  T qn[4], dxn[4], dyn[4], sn[4];
+  T l[4], r[4], gd[4], f[4];
+  cell_faces_from_q(a, i, j, q, dx, dy, s);
+  // x face between i-1 and i
+  cell_faces_from_q(a, i - 1, j, qn, dxn, dyn, sn);
+  face_state(a.hT, qn, dxn, sn, dtdx, true, l);
+  face_state(a.hT, q, dx, s, dtdx, false, r);
+  riemann<T, typename P::M, SOLVER>(a.hT, l, r, gd, f);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) a.fx[at(a.g, i, j, v)] = static_cast<U>(f[v]) * dtdxU;
+  // y face between j-1 and j
+  cell_faces_from_q(a, i, j - 1, qn, dxn, dyn, sn);
+  face_state(a.hT, qn, dyn, sn, dtdy, true, l);
+  face_state(a.hT, q, dy, s, dtdy, false, r);
+  riemann_y<T, typename P::M, SOLVER>(a.hT, l, r, f);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) a.fy[at(a.g, i, j, v)] = static_cast<U>(f[v]) * dtdyU;
+}
+
 // updateHydro over the interior (serial_ref/m_run.f90:504-513)
 template <class P>
 __global__ __launch_bounds__(256) void k_update(const MatArgs<P> a) {
@@ -629,14 +682,21 @@ static hipError_t materialised_step(const e2d_params& p, const DevBuffers& b, co
   a.slot_cur = c.slot_cur;
   a.dt_host = c.dt_host;
   const int S_ = p.nx + 4, J_ = p.ny + 4;
-  hipLaunchKernelGGL(k_prim<P>, dim3((S_ + 63) / 64, (J_ + 3) / 4), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(k_slopes<P>, dim3((S_ - 2 + 63) / 64, (J_ - 2 + 3) / 4), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(k_trace<P>, dim3((S_ - 2 + 63) / 64, (J_ - 2 + 3) / 4), dim3(256), 0, st, a);
   const dim3 fgrid((S_ - 3 + 63) / 64, (J_ - 3 + 3) / 4);
-  if (p.riemann_solver == E2D_RIEMANN_HLLC)
-    hipLaunchKernelGGL((k_flux<P, 2>), fgrid, dim3(256), 0, st, a);
-  else
-    hipLaunchKernelGGL((k_flux<P, 0>), fgrid, dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_prim<P>, dim3((S_ + 63) / 64, (J_ + 3) / 4), dim3(256), 0, st, a);
+  if (p.implementation_version == E2D_IMPL_FLUX_STORE) {
+    if (p.riemann_solver == E2D_RIEMANN_HLLC)
+      hipLaunchKernelGGL((k_flux_store<P, 2>), fgrid, dim3(256), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_flux_store<P, 0>), fgrid, dim3(256), 0, st, a);
+  } else {
+    hipLaunchKernelGGL(k_slopes<P>, dim3((S_ - 2 + 63) / 64, (J_ - 2 + 3) / 4), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_trace<P>, dim3((S_ - 2 + 63) / 64, (J_ - 2 + 3) / 4), dim3(256), 0, st, a);
+    if (p.riemann_solver == E2D_RIEMANN_HLLC)
+      hipLaunchKernelGGL((k_flux<P, 2>), fgrid, dim3(256), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_flux<P, 0>), fgrid, dim3(256), 0, st, a);
+  }
   hipLaunchKernelGGL(k_update<P>, dim3((p.nx + 63) / 64, (p.ny + 3) / 4), dim3(256), 0, st, a);
   return hipGetLastError();
 }

@@ -2,7 +2,7 @@
 !! (cuf/m_HydroParameters.cuf:17-158, serial_ref/m_run.f90:48-147).
 !!
 !! Namelist groups &RUN, &MESH, &HYDRO as in the reference and &OTHER with
-!! implementationVersion (0 fused, 1 materialised) plus this implementation's
+!! implementationVersion (0 fused, 1 materialised, 2 flux-store) plus this implementation's
 !! options: precision ('fp64' | 'fp32' | 'mixed'), problem ('implode' | 'diag'
 !! | 'bumps'), riemann_solver (0 two-shock, 2 HLLC), seed, device_loop
 !! (keep dt on the device between outputs) and vtk_format ('ascii' |
@@ -85,8 +85,8 @@ contains
        dy = (ymax - ymin) / ny
     end if
 
-    if ((implementationVersion /= 0) .and. (implementationVersion /= 1)) then
-       write(*,*) 'The implementation version parameter should be 0 or 1 !!!'
+    if ((implementationVersion /= 0) .and. (implementationVersion /= 1) .and. (implementationVersion /= 2)) then
+       write(*,*) 'The implementation version parameter should 0, 1 or 2 !!!'
        write(*,*) 'Check your namelist file, section OTHER'
        stop
     else

@@ -20,7 +20,7 @@ import numpy as np
 PKG = pathlib.Path(__file__).resolve().parent
 LIB_PATH = PKG / "lib" / "libeuler2d_hip.so"
 
-IMPL_FUSED, IMPL_MATERIALISED = 0, 1
+IMPL_FUSED, IMPL_MATERIALISED, IMPL_FLUX_STORE = 0, 1, 2
 PREC_FP64, PREC_FP32, PREC_MIXED = 0, 1, 2
 RIEMANN_APPROX, RIEMANN_HLLC = 0, 2
 PROBLEM_IMPLODE, PROBLEM_DIAG, PROBLEM_BUMPS = 0, 1, 2

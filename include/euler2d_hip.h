@@ -42,6 +42,7 @@ extern "C" {
 /* implementation_version: cuf/m_HydroConstants.cuf:51-53 */
 #define E2D_IMPL_FUSED 0        /* one fused kernel per step (LDS-free register marching) */
 #define E2D_IMPL_MATERIALISED 1 /* q, slopes, face states and fluxes stored in HBM */
+#define E2D_IMPL_FLUX_STORE 2   /* q and fluxes stored in HBM, slopes and face states in registers */
 
 /* precision */
 #define E2D_PREC_FP64 0   /* FP64 state and arithmetic (serial_ref built with fp_kind = dp_kind) */
@@ -57,7 +58,8 @@ extern "C" {
 #define E2D_PROBLEM_DIAG 1    /* x+y<1: rho=1,p=1 else rho=0.125,p=0.1; u=v=0 */
 #define E2D_PROBLEM_BUMPS 2   /* DIAG plus 16 seeded Gaussian bumps in rho and p */
 
-/* stages kept by the materialised implementation, for e2d_download_stage */
+/* stages kept by the materialised implementations, for e2d_download_stage
+ * (version 1: all nine; version 2: Q, FX and FY) */
 #define E2D_STAGE_Q 0
 #define E2D_STAGE_FX 1
 #define E2D_STAGE_FY 2
@@ -109,7 +111,7 @@ const char *e2d_last_error(void);
 /* Replaces initHydroParameters_gpu + the allocations of initHydroRun
  * (cuf/m_HydroParameters_gpu.cuf:67-103, cuf/m_HydroRun_gpu.cuf:83-120):
  * validates p, allocates the two state buffers (u = buffer 0, u2 = buffer 1)
- * and, for the materialised version, the stage arrays.  Buffer 1 starts at
+ * and, for implementation versions 1 and 2, the stage arrays.  Buffer 1 starts at
  * zero, as the reference's never-initialised u2 (serial_ref/m_run.f90:155). */
 int e2d_create(const e2d_params *p, e2d_ctx **out);
 
@@ -166,7 +168,7 @@ int e2d_set_invdt(e2d_ctx *ctx, double invdt);
  * in a fixed order (deterministic). */
 int e2d_sums(e2d_ctx *ctx, int buf, double out[4]);
 
-/* Materialised version only: download a stage array of the last step
+/* Implementation versions 1 and 2 only: download a stage array of the last step
  * (layout as the state; q, dq and face arrays (isize,jsize,4), fx/fy as the
  * reference's fx/fy arrays, serial_ref/m_run.f90:17). */
 int e2d_download_stage(e2d_ctx *ctx, int stage, double *h);

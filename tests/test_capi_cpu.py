@@ -41,7 +41,7 @@ def _params(**kw):
 
 
 @pytest.mark.parametrize("bad, msg", [
-    (dict(implementation_version=2), "implementation version"),
+    (dict(implementation_version=3), "implementation version"),
     (dict(implementation_version=7), "implementation version"),
     (dict(nx=0), "nx and ny"),
     (dict(ny=3), "nx and ny"),

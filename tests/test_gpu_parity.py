@@ -3,7 +3,7 @@
 Bar: FP64 and FP32 (the reference's two kinds) are bit-identical to the
 reference (golden fixtures from the compiled serial_ref, and the oracle on
 other sizes), up to the sign of zero; the mixed variant is checked against
-FP64 with a stated tolerance.  Both implementation versions, both loops (the
+FP64 with a stated tolerance.  All three implementation versions, both loops (the
 host-driven compute_dt + godunov_unsplit of serial_ref/main.f90:53-76, and the
 device-resident e2d_run_steps) are covered.
 """
@@ -70,7 +70,7 @@ RUNS = ["diag128_fp64", "implode128_fp64", "diag128_fp32", "implode64_fp32_mc",
         "implode50x30_fp64_mixedbc"]
 
 
-@pytest.mark.parametrize("impl", [0, 1])
+@pytest.mark.parametrize("impl", [0, 1, 2])
 @pytest.mark.parametrize("name", RUNS)
 def test_run_fixture_host_loop(gpu_lib, golden, name, impl):
     d, nx, ny, steps, ic, prec, kw = _fixture_case(golden, name)
@@ -83,7 +83,7 @@ def test_run_fixture_host_loop(gpu_lib, golden, name, impl):
     assert _same(u2, d["u2_end"]), _diff_report(u2, d["u2_end"])
 
 
-@pytest.mark.parametrize("impl", [0, 1])
+@pytest.mark.parametrize("impl", [0, 1, 2])
 @pytest.mark.parametrize("name", ["diag128_fp64", "diag128_fp32", "implode40x48_fp64_periodic"])
 def test_run_fixture_device_loop(gpu_lib, golden, name, impl):
     """e2d_run_steps keeps dt on the device; results equal the host loop's."""
@@ -108,11 +108,12 @@ def test_run_fixture_device_loop(gpu_lib, golden, name, impl):
     assert _same(u2, d["u2_end"]), _diff_report(u2, d["u2_end"])
 
 
-def test_stage_arrays_step0(gpu_lib, golden):
-    """implementationVersion 1 keeps q, fx, fy: equal to the reference's
+@pytest.mark.parametrize("impl", [1, 2])
+def test_stage_arrays_step0(gpu_lib, golden, impl):
+    """implementationVersions 1 and 2 keep q, fx, fy: equal to the reference's
     module arrays after step 0 (serial_ref/m_run.f90:14-17)."""
     d = np.load(golden / "run_diag128_fp64.npz")
-    with e2d.HydroRun(_params(128, impl=1)) as run:
+    with e2d.HydroRun(_params(128, impl=impl)) as run:
         run.init("diag")
         dt = run.compute_dt(0)
         run.make_boundaries(0)
@@ -122,6 +123,9 @@ def test_stage_arrays_step0(gpu_lib, golden):
         for st in ("q", "fx", "fy"):
             a = run.download_stage(st)
             assert _same(a, d[f"stage0_{st}"]), f"{st}: {_diff_report(a, d['stage0_' + st])}"
+        if impl == 2:
+            with pytest.raises(e2d.E2DError, match="only the q, fx and fy"):
+                run.download_stage("dqx")
         assert _same(run.download(1), d["snap_0"])
 
 
@@ -132,7 +136,7 @@ SIZES = [(4, 4, (1, 1, 1, 1)), (61, 17, (1, 2, 3, 1)), (121, 33, (3, 3, 3, 3)),
          (60, 16, (2, 2, 2, 2)), (7, 130, (1, 3, 2, 1))]
 
 
-@pytest.mark.parametrize("impl", [0, 1])
+@pytest.mark.parametrize("impl", [0, 1, 2])
 @pytest.mark.parametrize("nx,ny,bc", SIZES)
 def test_sizes_vs_oracle(gpu_lib, nx, ny, bc, impl):
     steps = 25
@@ -193,20 +197,21 @@ def test_2048_fp64_vs_oracle_first_steps(gpu_lib):
 
 
 def test_2048_fp64_fused_equals_materialised_and_symmetric(gpu_lib):
-    """Size-independent properties after 200 steps at 2048^2: the two
+    """Size-independent properties after 200 steps at 2048^2: the three
     implementation versions agree bit for bit, and the diagonal problem stays
     exactly transpose-symmetric (rho = rho^T, rho*u = (rho*v)^T)."""
     res = []
-    for impl in (0, 1):
+    for impl in (0, 1, 2):
         with e2d.HydroRun(_params(2048, impl=impl)) as run:
             run.init("diag")
             run.make_boundaries(0)
             run.run_steps(200)
             res.append((run.download(0), run.get_dt(), run.sums(0)))
-    (a, dta, sa), (b, dtb, sb) = res
-    assert dta == dtb
-    assert _same(a, b), _diff_report(a, b)
-    assert np.array_equal(sa, sb)
+    a, dta, sa = res[0]
+    for b, dtb, sb in res[1:]:
+        assert dta == dtb
+        assert _same(a, b), _diff_report(a, b)
+        assert np.array_equal(sa, sb)
     i = slice(2, 2050)
     assert _same(a[i, i, 0], a[i, i, 0].T)
     assert _same(a[i, i, 2], a[i, i, 3].T)

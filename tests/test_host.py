@@ -37,11 +37,12 @@ def test_usage_without_argument(exe, tmp_path):
     assert "Usage:" in r.stdout
 
 
-@pytest.mark.parametrize("version", [2, 7, -1])
+@pytest.mark.parametrize("version", [3, 7, -1])
 def test_bad_implementation_version(exe, tmp_path, version):
     write_nml(tmp_path / "t.nml", 16, extra=f"&OTHER\nimplementationVersion={version}\n/\n")
     r = _run(["t.nml"], tmp_path)
-    assert "The implementation version parameter should be 0 or 1" in r.stdout
+    # the reference's message, word for word (cuf/m_HydroParameters.cuf:130)
+    assert "The implementation version parameter should 0, 1 or 2 !!!" in r.stdout
     assert not list(tmp_path.glob("*.vti"))
 
 
@@ -91,7 +92,7 @@ def _read_vti_binary(path: pathlib.Path, nx: int, ny: int, dtype) -> np.ndarray:
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("kind", ["fp64", "fp32"])
-@pytest.mark.parametrize("impl", [0, 1])
+@pytest.mark.parametrize("impl", [0, 1, 2])
 def test_vtk_files_byte_identical_to_reference_program(gpu_lib, exe, tmp_path, kind, impl):
     write_nml(tmp_path / "test.nml", 16, 16, nstepmax=3, noutput=2, gamma0=1.4, slope_type=1,
               extra=f"&OTHER\nimplementationVersion={impl}\nprecision='{kind}'\n/\n")

@@ -137,11 +137,43 @@ def precision_error(e2d, nx, steps):
     return out
 
 
+def _rollout(e2d, nx, prec, dts, nsnap, lam_of):
+    """Forward with the frozen dt sequence, then the checkpointed adjoint of
+    lam_of(final state); returns (forward s, gradient s, recomputed steps,
+    gradient array, final state)."""
+    run = make_run(e2d, nx, 0, prec)
+    u0 = run.download(0)
+    run.synchronize()
+    t0 = time.perf_counter()
+    for n, d in enumerate(dts):
+        run.godunov_unsplit(n, d)
+    run.synchronize()
+    t_fwd = time.perf_counter() - t0
+    uT = run.download(len(dts) % 2)
+    lam_n, lam_0 = run.field_alloc(), run.field_alloc()
+    run.field_upload(lam_n, lam_of(uT))
+    run.upload(0, u0)
+    run.synchronize()
+    t0 = time.perf_counter()
+    nf = run.adjoint_rollout(0, dts, nsnap, lam_n, lam_0)
+    t_adj = time.perf_counter() - t0
+    grad = run.field_download(lam_0)
+    run.close()
+    return t_fwd, t_adj, nf, grad, uT
+
+
 def adjoint_metrics(e2d):
-    """Adjoint cost: one adjoint step against one forward step at 2048^2 FP64
-    (kernel time from hipEvents), and whole-gradient rollouts (forward pass with
-    the dt sequence recorded, then the binomial-checkpointed adjoint) for
-    512^2 x 400 steps with 20 snapshots and 2048^2 x 2000 steps with 45."""
+    """Adjoint cost and accuracy.
+    * one adjoint step against one forward step at 2048^2 FP64 (kernel time
+      from hipEvents);
+    * 512^2 x 400 steps, 20 snapshots, J = sum of rho over [0.6,0.8]^2 (FP64);
+    * 2048^2 x 2000 steps, 45 snapshots, J = sum over the interior of
+      (rho_T - rho_target)^2 with a seeded smooth target: FP64 and FP32
+      (forward and adjoint in float, lam accumulated in double), same frozen
+      dt sequence (recorded from the FP64 run); the FP32 gradient's relative
+      L2 error against the FP64 one.
+    Forward times are the step launches alone (dt frozen), gradient times the
+    whole checkpointed reverse sweep including recomputation."""
     out = {}
     run = make_run(e2d, 2048, 0, e2d.PREC_FP64)
     run.run_steps(100)
@@ -160,33 +192,56 @@ def adjoint_metrics(e2d):
     f_ms, a_ms = fwd[0] / fwd[1], adj[0] / adj[1]
     out["step_2048_fp64"] = {"forward_kernel_ms": round(f_ms, 4), "adjoint_kernels_ms": round(a_ms, 4),
                              "adjoint_over_forward": round(a_ms / f_ms, 3)}
-    for name, (nx, nsteps, nsnap) in {"rollout_512_400steps_20snap": (512, 400, 20),
-                                      "rollout_2048_2000steps_45snap": (2048, 2000, 45)}.items():
+
+    def record_dts(nx, nsteps):
         run = make_run(e2d, nx, 0, e2d.PREC_FP64)
-        u0 = run.download(0)
         dts = []
-        run.synchronize()
-        t0 = time.perf_counter()
         for n in range(nsteps):
             d = run.compute_dt(n % 2)
             dts.append(d)
             run.godunov_unsplit(n, d)
-        run.synchronize()
-        t_fwd = time.perf_counter() - t0
-        lam_n, lam_0 = run.field_alloc(), run.field_alloc()
-        mask = np.zeros((nx + 4, nx + 4, 4))
-        xc = (np.arange(nx + 4) - 2 + 0.5) / nx
-        sel = (xc >= 0.6) & (xc <= 0.8)
-        mask[np.ix_(sel, sel, [0])] = 1.0  # J = sum of rho over [0.6,0.8]^2
-        run.field_upload(lam_n, mask)
-        run.upload(0, u0)
-        run.synchronize()
-        t0 = time.perf_counter()
-        nf = run.adjoint_rollout(0, dts, nsnap, lam_n, lam_0)
-        t_adj = time.perf_counter() - t0
         run.close()
-        out[name] = {"forward_s": round(t_fwd, 3), "gradient_s": round(t_adj, 3),
-                     "gradient_over_forward": round(t_adj / t_fwd, 3), "forward_steps_recomputed": nf}
+        return dts
+
+    nx = 512
+    xc = (np.arange(nx + 4) - 2 + 0.5) / nx
+    sel = (xc >= 0.6) & (xc <= 0.8)
+    box = np.zeros((nx + 4, nx + 4, 4))
+    box[np.ix_(sel, sel, [0])] = 1.0
+    dts = record_dts(nx, 400)
+    t_f, t_a, nf, _, _ = _rollout(e2d, nx, e2d.PREC_FP64, dts, 20, lambda uT: box)
+    out["rollout_512_400steps_20snap_fp64"] = {"forward_s": round(t_f, 4), "gradient_s": round(t_a, 4),
+                                               "gradient_over_forward": round(t_a / t_f, 3),
+                                               "forward_steps_recomputed": nf}
+
+    nx, nsteps, nsnap = 2048, 2000, 45
+    X, Y = np.meshgrid((np.arange(nx + 4) - 1.5) / nx, (np.arange(nx + 4) - 1.5) / nx, indexing="ij")
+    rng = np.random.default_rng(2048)
+    k = rng.integers(1, 4, size=(3, 2))
+    ph = rng.uniform(size=(3, 2))
+    target = 1.0 + 0.3 * sum(np.sin(2 * np.pi * (k[m, 0] * X + ph[m, 0])) * np.cos(2 * np.pi * (k[m, 1] * Y + ph[m, 1]))
+                             for m in range(3)) / 3
+    inner = np.zeros((nx + 4, nx + 4), dtype=bool)
+    inner[2:nx + 2, 2:nx + 2] = True
+
+    def misfit_lam(uT):
+        lam = np.zeros_like(uT)
+        lam[..., 0] = np.where(inner, 2.0 * (uT[..., 0] - target), 0.0)
+        return lam
+
+    dts = record_dts(nx, nsteps)
+    c5, grads = {}, {}
+    for name, prec in (("fp64", e2d.PREC_FP64), ("fp32", e2d.PREC_FP32)):
+        t_f, t_a, nf, g, uT = _rollout(e2d, nx, prec, dts, nsnap, misfit_lam)
+        grads[name] = g[2:nx + 2, 2:nx + 2, :]
+        c5[name] = {"forward_s": round(t_f, 4), "gradient_s": round(t_a, 4),
+                    "gradient_over_forward": round(t_a / t_f, 3), "forward_steps_recomputed": nf,
+                    "J": float(np.sum(np.where(inner, (uT[..., 0] - target) ** 2, 0.0)))}
+    d = grads["fp32"] - grads["fp64"]
+    c5["fp32_gradient_rel_l2_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(grads["fp64"]))
+    c5["fp32_gradient_rel_l2_vs_fp64_per_variable"] = [
+        float(np.linalg.norm(d[..., v]) / np.linalg.norm(grads["fp64"][..., v])) for v in range(4)]
+    out["rollout_2048_2000steps_45snap_misfit"] = c5
     return out
 
 

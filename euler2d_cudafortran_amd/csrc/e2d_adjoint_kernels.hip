@@ -1,4 +1,4 @@
-// Adjoint of one Godunov step (FP64, frozen dt): lam_out = J^T lam_in, where J is
+// Adjoint of one Godunov step (frozen dt): lam_out = J^T lam_in, where J is
 // the Jacobian of the interior update u' = u + (fy(j)-fy(j+1)) + (fx(i)-fx(i+1))
 // (serial_ref/m_run.f90:504-513) with respect to the interior of u; ghost cells
 // are functions of the interior (make_boundaries), corners are constants.
@@ -15,6 +15,13 @@
 //                   then the primitive-variable adjoint -> bar of u (all cells)
 //   5. k_adj_fold   per interior cell: lam_in + bar of u + bars of the ghost
 //                   cells whose source it is
+//
+// The reverse passes run in the arithmetic type T and math policy of the
+// forward step's precision (double for fp64; float for fp32 and the mixed
+// variant) and recompute the forward values exactly as the step does, so every
+// branch decision matches it.  lam_in and lam_out are double for every
+// precision: the face flux bars are formed in double and rounded to T, and
+// each step's contribution is added to lam in double.
 #include <hip/hip_runtime.h>
 
 #include "e2d_adjoint.hpp"
@@ -25,49 +32,66 @@ namespace e2d {
 
 namespace {
 
-using MA = RangedMath;
-
+template <class P>
 struct AdjArgs {
-  Hydro<double> h;
+  using S = typename P::S;
+  using T = typename P::T;
+  using U = typename P::U;
+  Hydro<T> h;
+  Hydro<U> hU;
   Grid g;
-  const double* u;
+  const S* u;
   const double* lam;
   double* out;
-  AdjointScratch s;
-  double dtdx, dtdy;
+  T *q, *xminb, *xmaxb, *yminb, *ymaxb, *qb, *dqxb, *dqyb, *ub;
+  double dt;
 };
 
-__device__ __forceinline__ void load4(const double* a, const Grid& g, int i, int j, double x[4]) {
+template <class X, class T>
+__device__ __forceinline__ void load4(const X* a, const Grid& g, int i, int j, T x[4]) {
 #pragma unroll
-  for (int v = 0; v < 4; ++v) x[v] = a[at(g, i, j, v)];
+  for (int v = 0; v < 4; ++v) x[v] = static_cast<T>(a[at(g, i, j, v)]);
 }
 
-__device__ __forceinline__ void store4(double* a, const Grid& g, int i, int j, const double x[4]) {
+template <class T>
+__device__ __forceinline__ void store4(T* a, const Grid& g, int i, int j, const T x[4]) {
 #pragma unroll
   for (int v = 0; v < 4; ++v) a[at(g, i, j, v)] = x[v];
 }
 
-// slopes and trace sources of cell (i, j) from q of the cell and its neighbours
-__device__ __forceinline__ void cell_trace(const AdjArgs& a, int i, int j, double q[4], double dqx[4], double dqy[4],
-                                           double s[4]) {
-  double xp[4], xm[4], yp[4], ym[4];
-  load4(a.s.q, a.g, i, j, q);
-  load4(a.s.q, a.g, i + 1, j, xp);
-  load4(a.s.q, a.g, i - 1, j, xm);
-  load4(a.s.q, a.g, i, j + 1, yp);
-  load4(a.s.q, a.g, i, j - 1, ym);
-  cell_slopes(a.h, q, xp, xm, yp, ym, dqx, dqy);
-  trace_sources<double, MA>(a.h, q, dqx, dqy, s);
+// dt/dx and dt/dy as the forward step forms them (in the update type U)
+template <class P>
+__device__ __forceinline__ void step_ratios(const AdjArgs<P>& a, typename P::U& dtdx, typename P::U& dtdy) {
+  using U = typename P::U;
+  const U dt = static_cast<U>(a.dt);
+  dtdx = dt / a.hU.dx;
+  dtdy = dt / a.hU.dy;
 }
 
-__global__ __launch_bounds__(256) void k_adj_prim(const AdjArgs a) {
+// slopes and trace sources of cell (i, j) from q of the cell and its neighbours
+template <class P>
+__device__ __forceinline__ void cell_trace(const AdjArgs<P>& a, int i, int j, typename P::T q[4],
+                                           typename P::T dqx[4], typename P::T dqy[4], typename P::T s[4]) {
+  typename P::T xp[4], xm[4], yp[4], ym[4];
+  load4(a.q, a.g, i, j, q);
+  load4(a.q, a.g, i + 1, j, xp);
+  load4(a.q, a.g, i - 1, j, xm);
+  load4(a.q, a.g, i, j + 1, yp);
+  load4(a.q, a.g, i, j - 1, ym);
+  cell_slopes(a.h, q, xp, xm, yp, ym, dqx, dqy);
+  trace_sources<typename P::T, typename P::M>(a.h, q, dqx, dqy, s);
+}
+
+template <class P>
+__global__ __launch_bounds__(256) void k_adj_prim(const AdjArgs<P> a) {
+  using T = typename P::T;
   const int i = blockIdx.x * 64 + (threadIdx.x & 63);
   const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (i >= a.g.isize || j >= a.g.jsize) return;
-  double u[4], q[4];
+  T u[4], q[4];
   load4(a.u, a.g, i, j, u);
-  cons_to_prim<double, MA>(a.h, u, q);
-  store4(a.s.q, a.g, i, j, q);
+  cons_to_prim<T, typename P::M>(a.h, u, q);
+  store4(a.q, a.g, i, j, q);
 }
 
 __device__ __forceinline__ bool interior(const Grid& g, int i, int j) {
@@ -76,142 +100,159 @@ __device__ __forceinline__ bool interior(const Grid& g, int i, int j) {
 
 // faces: (i, j) in [2, nx+2] x [2, ny+2]; the x-face left of cell (i, j) for
 // interior rows, the y-face below cell (i, j) for interior columns
-__global__ __launch_bounds__(256) void k_adj_faces(const AdjArgs a) {
+template <class P>
+__global__ __launch_bounds__(256) void k_adj_faces(const AdjArgs<P> a) {
+  using T = typename P::T;
+  using U = typename P::U;
+  using M = typename P::M;
   const int i = 2 + blockIdx.x * 64 + (threadIdx.x & 63);
   const int j = 2 + blockIdx.y * 4 + (threadIdx.x >> 6);
   const Grid& g = a.g;
   if (i > g.nx + 2 || j > g.ny + 2) return;
+  U dtdxU, dtdyU;
+  step_ratios(a, dtdxU, dtdyU);
+  const T dtdx = static_cast<T>(dtdxU), dtdy = static_cast<T>(dtdyU);
   if (j <= g.ny + 1) {  // x-face between (i-1, j) and (i, j)
-    double fb[4];
+    T fb[4];
     bool any = false;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       double x = 0.0;
       if (interior(g, i, j)) x += a.lam[at(g, i, j, v)];
       if (interior(g, i - 1, j)) x -= a.lam[at(g, i - 1, j, v)];
-      fb[v] = x * a.dtdx;
+      fb[v] = static_cast<T>(x * static_cast<double>(dtdxU));
       any |= (x != 0.0);
     }
-    double lb[4] = {0.0, 0.0, 0.0, 0.0}, rb[4] = {0.0, 0.0, 0.0, 0.0};
+    T lb[4] = {T(0), T(0), T(0), T(0)}, rb[4] = {T(0), T(0), T(0), T(0)};
     if (any) {
-      double qL[4], dxL[4], dyL[4], sL[4], qR[4], dxR[4], dyR[4], sR[4], fl[4], fr[4];
+      T qL[4], dxL[4], dyL[4], sL[4], qR[4], dxR[4], dyR[4], sR[4], fl[4], fr[4];
       cell_trace(a, i - 1, j, qL, dxL, dyL, sL);
       cell_trace(a, i, j, qR, dxR, dyR, sR);
-      face_state(a.h, qL, dxL, sL, a.dtdx, true, fl);
-      face_state(a.h, qR, dxR, sR, a.dtdx, false, fr);
-      riemann_two_shock_adj<MA>(a.h, fl, fr, fb, lb, rb);
+      face_state(a.h, qL, dxL, sL, dtdx, true, fl);
+      face_state(a.h, qR, dxR, sR, dtdx, false, fr);
+      riemann_two_shock_adj<M>(a.h, fl, fr, fb, lb, rb);
     }
-    store4(a.s.xmaxb, g, i - 1, j, lb);
-    store4(a.s.xminb, g, i, j, rb);
+    store4(a.xmaxb, g, i - 1, j, lb);
+    store4(a.xminb, g, i, j, rb);
   }
   if (i <= g.nx + 1) {  // y-face between (i, j-1) and (i, j)
-    double fb[4];
+    T fb[4];
     bool any = false;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       double x = 0.0;
       if (interior(g, i, j)) x += a.lam[at(g, i, j, v)];
       if (interior(g, i, j - 1)) x -= a.lam[at(g, i, j - 1, v)];
-      fb[v] = x * a.dtdy;
+      fb[v] = static_cast<T>(x * static_cast<double>(dtdyU));
       any |= (x != 0.0);
     }
-    double lb[4] = {0.0, 0.0, 0.0, 0.0}, rb[4] = {0.0, 0.0, 0.0, 0.0};
+    T lb[4] = {T(0), T(0), T(0), T(0)}, rb[4] = {T(0), T(0), T(0), T(0)};
     if (any) {
-      double qL[4], dxL[4], dyL[4], sL[4], qR[4], dxR[4], dyR[4], sR[4], fl[4], fr[4];
+      T qL[4], dxL[4], dyL[4], sL[4], qR[4], dxR[4], dyR[4], sR[4], fl[4], fr[4];
       cell_trace(a, i, j - 1, qL, dxL, dyL, sL);
       cell_trace(a, i, j, qR, dxR, dyR, sR);
-      face_state(a.h, qL, dyL, sL, a.dtdy, true, fl);
-      face_state(a.h, qR, dyR, sR, a.dtdy, false, fr);
-      riemann_y_adj<MA>(a.h, fl, fr, fb, lb, rb);
+      face_state(a.h, qL, dyL, sL, dtdy, true, fl);
+      face_state(a.h, qR, dyR, sR, dtdy, false, fr);
+      riemann_y_adj<M>(a.h, fl, fr, fb, lb, rb);
     }
-    store4(a.s.ymaxb, g, i, j - 1, lb);
-    store4(a.s.yminb, g, i, j, rb);
+    store4(a.ymaxb, g, i, j - 1, lb);
+    store4(a.yminb, g, i, j, rb);
   }
 }
 
 // cells [1, size-2]^2: face-state and trace-source adjoints
-__global__ __launch_bounds__(256) void k_adj_cells(const AdjArgs a) {
+template <class P>
+__global__ __launch_bounds__(256) void k_adj_cells(const AdjArgs<P> a) {
+  using T = typename P::T;
+  using U = typename P::U;
   const int i = 1 + blockIdx.x * 64 + (threadIdx.x & 63);
   const int j = 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
   const Grid& g = a.g;
   if (i > g.isize - 2 || j > g.jsize - 2) return;
-  double bxm[4], bxp[4], bym[4], byp[4];
-  load4(a.s.xminb, g, i, j, bxm);
-  load4(a.s.xmaxb, g, i, j, bxp);
-  load4(a.s.yminb, g, i, j, bym);
-  load4(a.s.ymaxb, g, i, j, byp);
-  double qb[4] = {0.0, 0.0, 0.0, 0.0}, dxb[4] = {0.0, 0.0, 0.0, 0.0}, dyb[4] = {0.0, 0.0, 0.0, 0.0};
+  U dtdxU, dtdyU;
+  step_ratios(a, dtdxU, dtdyU);
+  const T dtdx = static_cast<T>(dtdxU), dtdy = static_cast<T>(dtdyU);
+  T bxm[4], bxp[4], bym[4], byp[4];
+  load4(a.xminb, g, i, j, bxm);
+  load4(a.xmaxb, g, i, j, bxp);
+  load4(a.yminb, g, i, j, bym);
+  load4(a.ymaxb, g, i, j, byp);
+  T qb[4] = {T(0), T(0), T(0), T(0)}, dxb[4] = {T(0), T(0), T(0), T(0)}, dyb[4] = {T(0), T(0), T(0), T(0)};
   bool any = false;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) any |= (bxm[v] != 0.0) | (bxp[v] != 0.0) | (bym[v] != 0.0) | (byp[v] != 0.0);
+  for (int v = 0; v < 4; ++v) any |= (bxm[v] != T(0)) | (bxp[v] != T(0)) | (bym[v] != T(0)) | (byp[v] != T(0));
   if (any) {
-    double q[4], dqx[4], dqy[4], s[4], sb[4] = {0.0, 0.0, 0.0, 0.0};
+    T q[4], dqx[4], dqy[4], s[4], sb[4] = {T(0), T(0), T(0), T(0)};
     cell_trace(a, i, j, q, dqx, dqy, s);
-    face_state_adj(a.h, q, dqx, s, a.dtdx, false, bxm, qb, dxb, sb);
-    face_state_adj(a.h, q, dqx, s, a.dtdx, true, bxp, qb, dxb, sb);
-    face_state_adj(a.h, q, dqy, s, a.dtdy, false, bym, qb, dyb, sb);
-    face_state_adj(a.h, q, dqy, s, a.dtdy, true, byp, qb, dyb, sb);
-    trace_sources_adj<MA>(a.h, q, dqx, dqy, sb, qb, dxb, dyb);
+    face_state_adj(a.h, q, dqx, s, dtdx, false, bxm, qb, dxb, sb);
+    face_state_adj(a.h, q, dqx, s, dtdx, true, bxp, qb, dxb, sb);
+    face_state_adj(a.h, q, dqy, s, dtdy, false, bym, qb, dyb, sb);
+    face_state_adj(a.h, q, dqy, s, dtdy, true, byp, qb, dyb, sb);
+    trace_sources_adj<typename P::M>(a.h, q, dqx, dqy, sb, qb, dxb, dyb);
   }
-  store4(a.s.qb, g, i, j, qb);
-  store4(a.s.dqxb, g, i, j, dxb);
-  store4(a.s.dqyb, g, i, j, dyb);
+  store4(a.qb, g, i, j, qb);
+  store4(a.dqxb, g, i, j, dxb);
+  store4(a.dqyb, g, i, j, dyb);
 }
 
 // all cells: bar of q from the cell's own trace and the limiters of the cell
 // and its four neighbours, then bar of u
-__global__ __launch_bounds__(256) void k_adj_gather(const AdjArgs a) {
+template <class P>
+__global__ __launch_bounds__(256) void k_adj_gather(const AdjArgs<P> a) {
+  using T = typename P::T;
   const int i = blockIdx.x * 64 + (threadIdx.x & 63);
   const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
   const Grid& g = a.g;
   if (i >= g.isize || j >= g.jsize) return;
-  double qb[4] = {0.0, 0.0, 0.0, 0.0};
+  T qb[4] = {T(0), T(0), T(0), T(0)};
   const bool cell_has = i >= 1 && i <= g.isize - 2 && j >= 1 && j <= g.jsize - 2;
-  if (cell_has) load4(a.s.qb, g, i, j, qb);
+  if (cell_has) load4(a.qb, g, i, j, qb);
   if (slopes_limited(a.h)) {
-    const double st = static_cast<double>(a.h.slope_type);
+    const T st = static_cast<T>(a.h.slope_type);
     // x-slopes of cells (i-1, j), (i, j), (i+1, j); y-slopes of (i, j-1), (i, j), (i, j+1)
     for (int dir = 0; dir < 2; ++dir) {
       for (int o = -1; o <= 1; ++o) {
         const int ni = dir == 0 ? i + o : i, nj = dir == 0 ? j : j + o;
         if (ni < 1 || ni > g.isize - 2 || nj < 1 || nj > g.jsize - 2) continue;
-        const double* bar = dir == 0 ? a.s.dqxb : a.s.dqyb;
+        const T* bar = dir == 0 ? a.dqxb : a.dqyb;
         const int mi = dir == 0 ? ni - 1 : ni, mj = dir == 0 ? nj : nj - 1;
         const int pi = dir == 0 ? ni + 1 : ni, pj = dir == 0 ? nj : nj + 1;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const double ob = bar[at(g, ni, nj, v)];
-          if (ob == 0.0) continue;
-          double bm = 0.0, b0 = 0.0, bp = 0.0;
-          limited_slope_adj(st, a.s.q[at(g, mi, mj, v)], a.s.q[at(g, ni, nj, v)], a.s.q[at(g, pi, pj, v)], ob, bm,
-                            b0, bp);
+          const T ob = bar[at(g, ni, nj, v)];
+          if (ob == T(0)) continue;
+          T bm = T(0), b0 = T(0), bp = T(0);
+          limited_slope_adj(st, a.q[at(g, mi, mj, v)], a.q[at(g, ni, nj, v)], a.q[at(g, pi, pj, v)], ob, bm, b0,
+                            bp);
           qb[v] += (o == 0) ? b0 : (o == 1 ? bm : bp);  // this cell is the neighbour's minus/plus side
         }
       }
     }
   }
-  double u[4], ub[4] = {0.0, 0.0, 0.0, 0.0};
+  T u[4], ub[4] = {T(0), T(0), T(0), T(0)};
   load4(a.u, g, i, j, u);
-  cons_to_prim_adj<MA>(a.h, u, qb, ub);
-  store4(a.s.ub, g, i, j, ub);
+  cons_to_prim_adj<typename P::M>(a.h, u, qb, ub);
+  store4(a.ub, g, i, j, ub);
 }
 
-// interior cells: lam_in + bar of u + bars of the ghost cells filled from this cell
-__global__ __launch_bounds__(256) void k_adj_fold(const AdjArgs a) {
+// interior cells: lam_in + bar of u + bars of the ghost cells filled from this
+// cell, summed in double
+template <class P>
+__global__ __launch_bounds__(256) void k_adj_fold(const AdjArgs<P> a) {
   const int i = 2 + blockIdx.x * 64 + (threadIdx.x & 63);
   const int j = 2 + blockIdx.y * 4 + (threadIdx.x >> 6);
   const Grid& g = a.g;
   if (i > g.nx + 1 || j > g.ny + 1) return;
   double r[4];
 #pragma unroll
-  for (int v = 0; v < 4; ++v) r[v] = a.lam[at(g, i, j, v)] + a.s.ub[at(g, i, j, v)];
+  for (int v = 0; v < 4; ++v) r[v] = a.lam[at(g, i, j, v)] + static_cast<double>(a.ub[at(g, i, j, v)]);
   const int gx[4] = {0, 1, g.nx + 2, g.nx + 3};
   for (int k = 0; k < 4; ++k) {  // x-face ghosts of row j
     const Src s = ghost_source(g, gx[k], j);
     if (s.i != i) continue;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const double b = a.s.ub[at(g, gx[k], j, v)];
+      const double b = static_cast<double>(a.ub[at(g, gx[k], j, v)]);
       r[v] += (v == s.flip) ? -b : b;
     }
   }
@@ -221,7 +262,7 @@ __global__ __launch_bounds__(256) void k_adj_fold(const AdjArgs a) {
     if (s.j != j) continue;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const double b = a.s.ub[at(g, i, gy[k], v)];
+      const double b = static_cast<double>(a.ub[at(g, i, gy[k], v)]);
       r[v] += (v == s.flip) ? -b : b;
     }
   }
@@ -229,28 +270,49 @@ __global__ __launch_bounds__(256) void k_adj_fold(const AdjArgs a) {
   for (int v = 0; v < 4; ++v) a.out[at(g, i, j, v)] = r[v];
 }
 
-}  // namespace
-
-hipError_t launch_adjoint_step(const e2d_params& p, const DevBuffers& b, int buf, double dt, const double* lam_in,
-                               double* lam_out, const AdjointScratch& s, hipStream_t st) {
-  AdjArgs a;
-  a.h = make_hydro<double>(p);
+template <class P>
+hipError_t adjoint_step(const e2d_params& p, const DevBuffers& b, int buf, double dt, const double* lam_in,
+                        double* lam_out, const AdjointScratch& s, hipStream_t st) {
+  using T = typename P::T;
+  AdjArgs<P> a;
+  a.h = make_hydro<T>(p);
+  a.hU = make_hydro<typename P::U>(p);
   a.g = make_grid(p, b);
-  a.u = static_cast<const double*>(b.u[buf]);
+  a.u = static_cast<const typename P::S*>(b.u[buf]);
   a.lam = lam_in;
   a.out = lam_out;
-  a.s = s;
-  a.dtdx = dt / a.h.dx;
-  a.dtdy = dt / a.h.dy;
+  a.q = static_cast<T*>(s.q);
+  a.xminb = static_cast<T*>(s.xminb);
+  a.xmaxb = static_cast<T*>(s.xmaxb);
+  a.yminb = static_cast<T*>(s.yminb);
+  a.ymaxb = static_cast<T*>(s.ymaxb);
+  a.qb = static_cast<T*>(s.qb);
+  a.dqxb = static_cast<T*>(s.dqxb);
+  a.dqyb = static_cast<T*>(s.dqyb);
+  a.ub = static_cast<T*>(s.ub);
+  a.dt = dt;
   hipError_t e = launch_boundaries(p, b, buf, st);
   if (e != hipSuccess) return e;
   const Grid& g = a.g;
-  hipLaunchKernelGGL(k_adj_prim, dim3((g.isize + 63) / 64, (g.jsize + 3) / 4), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(k_adj_faces, dim3((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(k_adj_cells, dim3((g.isize - 2 + 63) / 64, (g.jsize - 2 + 3) / 4), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(k_adj_gather, dim3((g.isize + 63) / 64, (g.jsize + 3) / 4), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(k_adj_fold, dim3((g.nx + 63) / 64, (g.ny + 3) / 4), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_adj_prim<P>, dim3((g.isize + 63) / 64, (g.jsize + 3) / 4), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_adj_faces<P>, dim3((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_adj_cells<P>, dim3((g.isize - 2 + 63) / 64, (g.jsize - 2 + 3) / 4), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_adj_gather<P>, dim3((g.isize + 63) / 64, (g.jsize + 3) / 4), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_adj_fold<P>, dim3((g.nx + 63) / 64, (g.ny + 3) / 4), dim3(256), 0, st, a);
   return hipGetLastError();
+}
+
+}  // namespace
+
+size_t adjoint_scratch_elem_bytes(int precision) { return precision == E2D_PREC_FP64 ? sizeof(double) : sizeof(float); }
+
+hipError_t launch_adjoint_step(const e2d_params& p, const DevBuffers& b, int buf, double dt, const double* lam_in,
+                               double* lam_out, const AdjointScratch& s, hipStream_t st) {
+  switch (p.precision) {
+    case E2D_PREC_FP64: return adjoint_step<P64>(p, b, buf, dt, lam_in, lam_out, s, st);
+    case E2D_PREC_FP32: return adjoint_step<P32>(p, b, buf, dt, lam_in, lam_out, s, st);
+    default: return adjoint_step<PMix>(p, b, buf, dt, lam_in, lam_out, s, st);
+  }
 }
 
 }  // namespace e2d

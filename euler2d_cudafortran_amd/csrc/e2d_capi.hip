@@ -170,9 +170,9 @@ size_t field_bytes(const e2d_ctx* c) { return static_cast<size_t>(c->b.plane) * 
 void free_adjoint(e2d_ctx* c) {
   for (auto& d : c->dual)
     if (d) (void)hipFree(d), d = nullptr;
-  double** arrs[] = {&c->adj.q, &c->adj.xminb, &c->adj.xmaxb, &c->adj.yminb, &c->adj.ymaxb,
-                     &c->adj.qb, &c->adj.dqxb, &c->adj.dqyb, &c->adj.ub};
-  for (double** a : arrs)
+  void** arrs[] = {&c->adj.q, &c->adj.xminb, &c->adj.xmaxb, &c->adj.yminb, &c->adj.ymaxb,
+                   &c->adj.qb, &c->adj.dqxb, &c->adj.dqyb, &c->adj.ub};
+  for (void** a : arrs)
     if (*a) (void)hipFree(*a), *a = nullptr;
   for (void* sn : c->snapshots) (void)hipFree(sn);
   c->snapshots.clear();
@@ -491,9 +491,10 @@ int e2d_device_buffer(e2d_ctx* c, int buf, void** dptr, int64_t* pitch) {
 
 namespace {
 
-int check_linearisable(const e2d_ctx* c) {
-  if (c->p.precision != E2D_PREC_FP64)
-    return fail(E2D_ERR_STATE, "tangent and adjoint need precision 0 (fp64)");
+// the adjoint exists for every precision, the tangent (dual numbers) for fp64
+int check_linearisable(const e2d_ctx* c, bool tangent) {
+  if (tangent && c->p.precision != E2D_PREC_FP64)
+    return fail(E2D_ERR_STATE, "the tangent step needs precision 0 (fp64)");
   if (c->p.riemann_solver != E2D_RIEMANN_APPROX)
     return fail(E2D_ERR_STATE, "tangent and adjoint need the two-shock solver (riemann_solver 0)");
   return E2D_OK;
@@ -501,15 +502,16 @@ int check_linearisable(const e2d_ctx* c) {
 
 int ensure_adjoint_scratch(e2d_ctx* c) {
   if (c->adj.q) return E2D_OK;
-  double** arrs[] = {&c->adj.q, &c->adj.xminb, &c->adj.xmaxb, &c->adj.yminb, &c->adj.ymaxb,
-                     &c->adj.qb, &c->adj.dqxb, &c->adj.dqyb, &c->adj.ub};
-  for (double** a : arrs) {
-    if (hipMalloc(reinterpret_cast<void**>(a), field_bytes(c)) != hipSuccess) {
+  const size_t bytes = static_cast<size_t>(c->b.plane) * 4 * e2d::adjoint_scratch_elem_bytes(c->p.precision);
+  void** arrs[] = {&c->adj.q, &c->adj.xminb, &c->adj.xmaxb, &c->adj.yminb, &c->adj.ymaxb,
+                   &c->adj.qb, &c->adj.dqxb, &c->adj.dqyb, &c->adj.ub};
+  for (void** a : arrs) {
+    if (hipMalloc(a, bytes) != hipSuccess) {
       free_adjoint(c);
       return fail(E2D_ERR_NOMEM, "adjoint scratch allocation failed");
     }
     // slots never written by a pass must read as zero
-    HIP_TRY(hipMemsetAsync(*a, 0, field_bytes(c), c->stream));
+    HIP_TRY(hipMemsetAsync(*a, 0, bytes, c->stream));
   }
   return E2D_OK;
 }
@@ -597,7 +599,7 @@ struct Rollout {
 int e2d_tangent_step(e2d_ctx* c, int buf, double dt, const double* v_in, double* v_out) {
   if (int rc = check_ctx(c)) return rc;
   if (int rc = check_buf(buf)) return rc;
-  if (int rc = check_linearisable(c)) return rc;
+  if (int rc = check_linearisable(c, true)) return rc;
   if (!v_in || !v_out) return fail(E2D_ERR_ARG, "field pointer is NULL");
   if (!(dt > 0.0)) return fail(E2D_ERR_ARG, "dt must be > 0");
   for (auto& d : c->dual)
@@ -612,7 +614,7 @@ int e2d_tangent_step(e2d_ctx* c, int buf, double dt, const double* v_in, double*
 int e2d_adjoint_step(e2d_ctx* c, int buf, double dt, const double* lam_in, double* lam_out) {
   if (int rc = check_ctx(c)) return rc;
   if (int rc = check_buf(buf)) return rc;
-  if (int rc = check_linearisable(c)) return rc;
+  if (int rc = check_linearisable(c, false)) return rc;
   if (!lam_in || !lam_out) return fail(E2D_ERR_ARG, "field pointer is NULL");
   if (!(dt > 0.0)) return fail(E2D_ERR_ARG, "dt must be > 0");
   if (int rc = ensure_adjoint_scratch(c)) return rc;
@@ -622,7 +624,7 @@ int e2d_adjoint_step(e2d_ctx* c, int buf, double dt, const double* lam_in, doubl
 int e2d_adjoint_rollout(e2d_ctx* c, int nStep0, int nsteps, const double* dts, int nsnap, const double* lam_n,
                         double* lam_0, int64_t* n_forward) {
   if (int rc = check_ctx(c)) return rc;
-  if (int rc = check_linearisable(c)) return rc;
+  if (int rc = check_linearisable(c, false)) return rc;
   if (nStep0 < 0 || nsteps < 1) return fail(E2D_ERR_ARG, "need nStep0 >= 0 and nsteps >= 1");
   if (nsnap < 1) return fail(E2D_ERR_ARG, "need at least one snapshot");
   if (!dts || !lam_n || !lam_0) return fail(E2D_ERR_ARG, "NULL argument");

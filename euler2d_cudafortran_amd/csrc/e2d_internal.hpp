@@ -66,10 +66,13 @@ hipError_t launch_pointwise(const e2d_params& p, int func, int n, const double* 
 hipError_t launch_tangent_step(const e2d_params& p, const DevBuffers& b, int buf, double dt, const double* v_in,
                                double* v_out, void* dual_in, void* dual_out, hipStream_t s);
 // adjoint step at the state of buffer buf (FP64): lam_out = J^T lam_in, dt frozen
+// per-cell scratch of the adjoint step, in the step's arithmetic type
+// (adjoint_scratch_elem_bytes bytes per value, plane * 4 values per array)
 struct AdjointScratch {
-  double *q = nullptr, *xminb = nullptr, *xmaxb = nullptr, *yminb = nullptr, *ymaxb = nullptr;
-  double *qb = nullptr, *dqxb = nullptr, *dqyb = nullptr, *ub = nullptr;
+  void *q = nullptr, *xminb = nullptr, *xmaxb = nullptr, *yminb = nullptr, *ymaxb = nullptr;
+  void *qb = nullptr, *dqxb = nullptr, *dqyb = nullptr, *ub = nullptr;
 };
+size_t adjoint_scratch_elem_bytes(int precision);
 hipError_t launch_adjoint_step(const e2d_params& p, const DevBuffers& b, int buf, double dt,
                                const double* lam_in, double* lam_out, const AdjointScratch& a, hipStream_t s);
 

@@ -197,12 +197,15 @@ int e2d_device_buffer(e2d_ctx *ctx, int buf, void **dptr, int64_t *pitch);
 
 /*
  * Tangent-linear and adjoint of the step (no counterpart in the reference).
- * FP64 and the two-shock solver only.  The step is linearised as executed
- * (floors, limiter branches, sampling branches and Newton iteration counts of
- * the forward pass, |x|' = +1 at 0) with dt frozen.  Vector fields are FP64
- * device arrays in the state's device layout (element (i,j,v) at
- * [(v*jsize + j)*pitch + i], see e2d_device_buffer); only interior entries
- * are read and written.
+ * Two-shock solver only; the tangent for FP64, the adjoint for every
+ * precision (its reverse passes run in the step's arithmetic type, FP32 for
+ * the FP32 and mixed variants, and add each step's contribution to lam in
+ * FP64).  The step is linearised as executed (floors, limiter branches,
+ * sampling branches and Newton iteration counts of the forward pass,
+ * |x|' = +1 at 0) with dt frozen.  Vector fields are FP64 device arrays in
+ * the state's device layout (element (i,j,v) at [(v*jsize + j)*pitch + i],
+ * see e2d_device_buffer) for every precision; only interior entries are read
+ * and written.
  */
 
 /* v_out = J v_in for the step whose input state is buffer buf (the state is

@@ -5,9 +5,11 @@
 // dual-number tangent (<J v, w> = <v, J^T w>, relative error printed), and the
 // tangent against a central finite difference.  Prints one line per routine:
 //   <name> n=<cases> dot=<max rel err> fd=<max rel err>
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <random>
+#include <vector>
 
 #include "../euler2d_cudafortran_amd/csrc/e2d_adjoint.hpp"
 
@@ -143,7 +145,7 @@ int main() {
       const Dual o = limited_slope<Dual>(Dual(double(stype)), Dual(qm, v0), Dual(q0, v1), Dual(qp, v2));
       const double jv_w = o.d * w;
       double a = 0, b = 0, cc = 0;
-      limited_slope_adj(stype, qm, q0, qp, w, a, b, cc);
+      limited_slope_adj(double(stype), qm, q0, qp, w, a, b, cc);
       const double v_jtw = v0 * a + v1 * b + v2 * cc;
       st.dot = std::fmax(st.dot, std::fabs(jv_w - v_jtw) / std::fmax(1.0, std::fabs(jv_w)));
       st.n++;
@@ -199,6 +201,59 @@ int main() {
       st.n++;
     }
     std::printf("trace n=%d dot=%.3e fd=0\n", st.n, st.dot);
+  }
+
+  // ---- the float instantiation (FP32 and mixed adjoints) against the double
+  // one on the same float inputs: 99th percentile of the relative difference
+  // of <v, J^T w> (cases where the float forward takes another branch, e.g.
+  // one Newton iteration more or less, make up the tail)
+  {
+    Hydro<float> hf;
+    hf.gamma0 = static_cast<float>(h.gamma0);
+    hf.gamma6 = static_cast<float>(h.gamma6);
+    hf.cfl = static_cast<float>(h.cfl);
+    hf.smallr = static_cast<float>(h.smallr);
+    hf.smallc = static_cast<float>(h.smallc);
+    hf.smallp = static_cast<float>(h.smallp);
+    hf.smallpp = static_cast<float>(h.smallpp);
+    hf.entho = static_cast<float>(h.entho);
+    hf.dx = hf.dy = static_cast<float>(h.dx);
+    hf.conv_tol = 1e-6f;
+    hf.slope_type = h.slope_type;
+    hf.niter = h.niter;
+    hf.riemann_solver = 0;
+    std::vector<double> errs;
+    for (int c = 0; c < 4000; ++c) {
+      float ql[4], qr[4], w[4];
+      double qld[4], qrd[4], wd[4], v[8];
+      ql[0] = static_cast<float>(std::exp(std::log(0.05) + U(rng) * std::log(40.0)));
+      qr[0] = static_cast<float>(std::exp(std::log(0.05) + U(rng) * std::log(40.0)));
+      ql[1] = static_cast<float>(std::exp(std::log(0.05) + U(rng) * std::log(40.0)));
+      qr[1] = static_cast<float>(std::exp(std::log(0.05) + U(rng) * std::log(40.0)));
+      for (int k = 2; k < 4; ++k) {
+        ql[k] = static_cast<float>(2.0 * N(rng));
+        qr[k] = static_cast<float>(2.0 * N(rng));
+      }
+      for (int k = 0; k < 4; ++k) {
+        w[k] = static_cast<float>(N(rng));
+        qld[k] = ql[k];
+        qrd[k] = qr[k];
+        wd[k] = w[k];
+      }
+      for (double& x : v) x = N(rng);
+      float lb[4] = {0, 0, 0, 0}, rb[4] = {0, 0, 0, 0};
+      double lbd[4] = {0, 0, 0, 0}, rbd[4] = {0, 0, 0, 0};
+      riemann_two_shock_adj<M>(hf, ql, qr, w, lb, rb);
+      riemann_two_shock_adj<M>(h, qld, qrd, wd, lbd, rbd);
+      double a = 0, b = 0;
+      for (int k = 0; k < 4; ++k) {
+        a += v[k] * lb[k] + v[4 + k] * rb[k];
+        b += v[k] * lbd[k] + v[4 + k] * rbd[k];
+      }
+      errs.push_back(std::fabs(a - b) / std::fmax(1e-30, std::fabs(b)));
+    }
+    std::sort(errs.begin(), errs.end());
+    std::printf("riemann_two_shock_f32 n=%zu dot=%.3e fd=0\n", errs.size(), errs[errs.size() * 99 / 100]);
   }
   return 0;
 }

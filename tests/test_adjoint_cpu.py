@@ -44,3 +44,12 @@ def test_tangent_against_finite_differences(report, name):
     # worst case over 4000 random states, eps = 1e-6: states that sit within
     # eps of a branch switch (Newton count, sampling) dominate the maximum
     assert fd <= 1e-4, f"{name}: tangent vs finite difference {fd:.3e}"
+
+
+def test_float_instantiation_against_double(report):
+    # the FP32/mixed adjoint runs the same reverse code in float; on float inputs
+    # it must agree with the double instantiation to FP32 rounding (99th
+    # percentile; the tail is float forward passes taking another branch)
+    n, p99, _ = report["riemann_two_shock_f32"]
+    assert n >= 4000
+    assert p99 <= 1e-4, f"float vs double Riemann adjoint, 99th percentile {p99:.3e}"

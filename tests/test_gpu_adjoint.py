@@ -290,3 +290,87 @@ def test_c4_adjoint_512(gpu_lib):
     rel = abs(fd - ad) / abs(ad)
     print(f"C4 gradient: adjoint {ad:.15e}, finite difference {fd:.15e}, rel {rel:.2e}")
     assert rel <= 1e-7
+
+
+def _adjoint_step_all_precisions(U, nx, dt, lam, **kw):
+    out = {}
+    for prec in (e2d.PREC_FP64, e2d.PREC_FP32, e2d.PREC_MIXED):
+        with e2d.HydroRun(_params(nx, precision=prec, **kw)) as run:
+            run.init("diag")
+            run.upload(0, U)
+            a, b = run.field_alloc(), run.field_alloc()
+            run.field_upload(a, lam)
+            run.adjoint_step(0, dt, a, b)
+            out[prec] = run.field_download(b)
+    return out
+
+
+def test_fp32_adjoint_step_against_fp64_first_order(gpu_lib):
+    """The FP32 and mixed adjoints (reverse passes in float, lam accumulated in
+    double) against the FP64 adjoint on the same float-representable smooth
+    state with flow, first-order reconstruction (slope_type 0, so no limiter
+    subgradient can flip): they differ by FP32 rounding only."""
+    nx = 96
+    mask = _interior(nx, nx)
+    U = _smooth_state(nx, nx, 41).astype(np.float32).astype(np.float64)
+    lam = np.where(mask, np.random.default_rng(42).normal(size=mask.shape), 0.0)
+    out = _adjoint_step_all_precisions(U, nx, 2e-3, lam, slope_type=0, boundary_type_xmin=3, boundary_type_xmax=3,
+                                       boundary_type_ymin=3, boundary_type_ymax=3)
+    ref = out[e2d.PREC_FP64][mask]
+    for prec in (e2d.PREC_FP32, e2d.PREC_MIXED):
+        err = np.linalg.norm(out[prec][mask] - ref) / np.linalg.norm(ref)
+        print(f"precision {prec}: adjoint step vs fp64, relative L2 {err:.2e}")
+        assert err <= 1e-5
+
+
+def test_fp32_adjoint_step_against_fp64_minmod(gpu_lib):
+    """Same comparison on the benchmark configuration (diagonal problem after
+    30 FP32 steps, minmod): where |dl| and |dr| of the limiter differ by less
+    than FP32 resolves (nearly linear profiles), the float forward takes the
+    other subgradient, so the adjoints differ beyond rounding (measured
+    relative L2 2.9e-3 at 128^2)."""
+    nx = 128
+    mask = _interior(nx, nx)
+    with e2d.HydroRun(_params(nx, precision=e2d.PREC_FP32)) as run:
+        U = _evolved_diag(run, 30)
+    lam = np.where(mask, np.random.default_rng(3).normal(size=mask.shape), 0.0)
+    out = _adjoint_step_all_precisions(U, nx, 1e-3, lam)
+    ref = out[e2d.PREC_FP64][mask]
+    for prec in (e2d.PREC_FP32, e2d.PREC_MIXED):
+        err = np.linalg.norm(out[prec][mask] - ref) / np.linalg.norm(ref)
+        print(f"precision {prec}: adjoint step vs fp64 (minmod, diagonal problem), relative L2 {err:.2e}")
+        assert err <= 1e-2
+
+
+def test_fp32_rollout_against_fp64(gpu_lib):
+    """C5 at reduced size (256^2, 200 steps, 10 snapshots): gradient of
+    J = sum over the interior of (rho_T - rho_target)^2 from the FP32 and mixed
+    forward/adjoint against the FP64 one, same frozen dt sequence (measured
+    relative L2 8.3e-4 / 8.4e-4)."""
+    nx, nsteps = 256, 200
+    mask = _interior(nx, nx)
+    X, Y = np.meshgrid((np.arange(nx + 4) - 1.5) / nx, (np.arange(nx + 4) - 1.5) / nx, indexing="ij")
+    target = 1 + 0.3 * np.sin(2 * np.pi * (X + 0.1)) * np.cos(2 * np.pi * (2 * Y + 0.3))
+    grads, dts = {}, None
+    for prec in (e2d.PREC_FP64, e2d.PREC_FP32, e2d.PREC_MIXED):
+        with e2d.HydroRun(_params(nx, precision=prec)) as run:
+            run.init("diag")
+            U0 = run.download(0)
+            if dts is None:
+                dts = _record_forward(run, U0, nsteps)
+            run.upload(0, U0)
+            for n in range(nsteps):
+                run.godunov_unsplit(n, dts[n])
+            uT = run.download(nsteps % 2)
+            lam = np.zeros_like(uT)
+            lam[..., 0] = np.where(mask[..., 0], 2 * (uT[..., 0] - target), 0.0)
+            a, b = run.field_alloc(), run.field_alloc()
+            run.field_upload(a, lam)
+            run.upload(0, U0)
+            run.adjoint_rollout(0, dts, 10, a, b)
+            grads[prec] = run.field_download(b)[mask]
+    ref = grads[e2d.PREC_FP64]
+    for prec in (e2d.PREC_FP32, e2d.PREC_MIXED):
+        err = np.linalg.norm(grads[prec] - ref) / np.linalg.norm(ref)
+        print(f"precision {prec}: rollout gradient vs fp64, relative L2 {err:.2e}")
+        assert err <= 1e-2

@@ -117,6 +117,26 @@ def variant(e2d, nx, impl, prec, steps, warmup):
             "ms_per_step": round(1e3 * wall / steps, 4), "kernel_ms": round(kms, 4)}
 
 
+def host_loop(e2d, nx, steps, warmup=10):
+    """The reference's loop shape: compute_dt (dt read back to the host) then
+    godunov_unsplit, every step."""
+    run = make_run(e2d, nx, 0, e2d.PREC_FP64)
+    n = 0
+    for _ in range(warmup):
+        run.godunov_unsplit(n, run.compute_dt(n % 2))
+        n += 1
+    run.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run.godunov_unsplit(n, run.compute_dt(n % 2))
+        n += 1
+    run.synchronize()
+    wall = time.perf_counter() - t0
+    run.close()
+    return {"mcell_per_s": round(nx * nx * steps / wall / 1e6, 1), "steps": steps,
+            "ms_per_step": round(1e3 * wall / steps, 4)}
+
+
 def precision_error(e2d, nx, steps):
     """FP32 variants against the FP64 run on the bumps problem: relative L1 and
     max-abs error per conserved variable after `steps` steps."""
@@ -318,6 +338,8 @@ def main(argv=None):
         }.items():
             _log(f"[bench] variant {name}")
             variants[name] = variant(e2d, vnx, impl, vprec, vsteps, 10)
+        _log("[bench] variant fp64_host_loop_2048")
+        variants["fp64_host_loop_2048"] = host_loop(e2d, 2048, 200)
         _log("[bench] precision error, bumps problem 4096^2, 500 steps")
         variants["precision_error_bumps_4096_500steps_vs_fp64"] = precision_error(e2d, 4096, 500)
 

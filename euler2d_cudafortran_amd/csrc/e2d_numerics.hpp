@@ -12,6 +12,8 @@
 // (serial_ref/m_constants.f90:11-14).
 #pragma once
 
+#include <type_traits>
+
 // Host and device: the kernels use these routines; the adjoint's checks also
 // compile them for the CPU.
 #if defined(__HIPCC__)
@@ -89,12 +91,32 @@ struct RangedMath {
 };
 #endif
 
-// Fortran max/min as the reference compiler lowers them (first operand wins
-// only when strictly greater / smaller; this fixes the sign of zero results).
+// Fortran max/min as the reference compiler lowers them: a > b ? a : b and
+// a < b ? a : b.  In double device code they are the hardware v_max_f64 /
+// v_min_f64 (IEEE maxNum/minNum), which return the same bits except when b is
+// NaN or {a, b} = {+0, -0}.  Neither case changes a result on the FP64 path:
+// every floor compares against a positive constant or rho*smallp > 0, the
+// limiter compares absolute values, max(x, 0) for the first pressure guess
+// never sees x = -0 (a sum with a positive term), and the Newton correction
+// max(t, -p) can meet {+0, -0} only to add a zero of either sign to p = +0.
+// NaN enters only with a NaN state.  E2D_SELECT_MINMAX=1 keeps the selects.
+#ifndef E2D_SELECT_MINMAX
+#define E2D_SELECT_MINMAX 0
+#endif
 template <class T>
-E2D_HDI T fmax2(T a, T b) { return a > b ? a : b; }
+E2D_HDI T fmax2(T a, T b) {
+#if defined(__HIP_DEVICE_COMPILE__) && !E2D_SELECT_MINMAX
+  if constexpr (std::is_same<T, double>::value) return __builtin_fmax(a, b);
+#endif
+  return a > b ? a : b;
+}
 template <class T>
-E2D_HDI T fmin2(T a, T b) { return a < b ? a : b; }
+E2D_HDI T fmin2(T a, T b) {
+#if defined(__HIP_DEVICE_COMPILE__) && !E2D_SELECT_MINMAX
+  if constexpr (std::is_same<T, double>::value) return __builtin_fmin(a, b);
+#endif
+  return a < b ? a : b;
+}
 template <class T>
 E2D_HDI T fabs_(T a) { return __builtin_fabs(a); }
 template <>

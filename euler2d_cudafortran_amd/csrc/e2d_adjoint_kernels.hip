@@ -12,9 +12,10 @@
 //   3. k_adj_cells  per cell: face-state and trace-source adjoints -> bars of
 //                   q, dq_x, dq_y of the cell
 //   4. k_adj_gather per cell: limiter adjoints of the cell and its 4 neighbours,
-//                   then the primitive-variable adjoint -> bar of u (all cells)
-//   5. k_adj_fold   per interior cell: lam_in + bar of u + bars of the ghost
-//                   cells whose source it is
+//                   then the primitive-variable adjoint -> bar of u; interior
+//                   cells write lam_in + that bar, ghost cells keep their bar
+//   5. k_adj_fold_frame  per cell of the 2-cell interior frame: + the bars of
+//                   the ghost cells whose source it is
 //
 // The reverse passes run in the arithmetic type T and math policy of the
 // forward step's precision (double for fp64; float for fp32 and the mixed
@@ -232,20 +233,48 @@ __global__ __launch_bounds__(256) void k_adj_gather(const AdjArgs<P> a) {
   T u[4], ub[4] = {T(0), T(0), T(0), T(0)};
   load4(a.u, g, i, j, u);
   cons_to_prim_adj<typename P::M>(a.h, u, qb, ub);
-  store4(a.ub, g, i, j, ub);
+  if (interior(g, i, j)) {
+    // lam_in + own bar of u, in double; ghost bars are added by k_adj_fold_frame
+#pragma unroll
+    for (int v = 0; v < 4; ++v) a.out[at(g, i, j, v)] = a.lam[at(g, i, j, v)] + static_cast<double>(ub[v]);
+  } else {
+    store4(a.ub, g, i, j, ub);
+  }
 }
 
-// interior cells: lam_in + bar of u + bars of the ghost cells filled from this
-// cell, summed in double
+// Ghost cells are filled from interior cells of columns {2, 3, nx, nx+1} or
+// rows {2, 3, ny, ny+1} (0-based; reflective, absorbing and periodic alike),
+// so only this frame receives ghost bars.  Thread t of the frame: rows
+// {2, 3, ny, ny+1} over all interior columns first, then columns
+// {2, 3, nx, nx+1} over the remaining rows [4, ny-1].
+__device__ __forceinline__ bool frame_cell(const Grid& g, int t, int& i, int& j) {
+  const int top = 4 * g.nx;
+  if (t < top) {
+    const int k = t / g.nx;
+    i = 2 + t % g.nx;
+    j = k < 2 ? 2 + k : g.ny + k - 2;
+    return true;
+  }
+  t -= top;
+  const int rows = g.ny - 4;
+  if (rows <= 0 || t >= 4 * rows) return false;
+  const int k = t / rows;
+  j = 4 + t % rows;
+  i = k < 2 ? 2 + k : g.nx + k - 2;
+  return true;
+}
+
+// frame cells: add the bars of the ghost cells filled from this cell to
+// lam_in + own bar (written by k_adj_gather), x-face ghosts of the row first,
+// then y-face ghosts of the column, all in double
 template <class P>
-__global__ __launch_bounds__(256) void k_adj_fold(const AdjArgs<P> a) {
-  const int i = 2 + blockIdx.x * 64 + (threadIdx.x & 63);
-  const int j = 2 + blockIdx.y * 4 + (threadIdx.x >> 6);
+__global__ __launch_bounds__(256) void k_adj_fold_frame(const AdjArgs<P> a) {
   const Grid& g = a.g;
-  if (i > g.nx + 1 || j > g.ny + 1) return;
+  int i, j;
+  if (!frame_cell(g, blockIdx.x * 256 + threadIdx.x, i, j)) return;
   double r[4];
 #pragma unroll
-  for (int v = 0; v < 4; ++v) r[v] = a.lam[at(g, i, j, v)] + static_cast<double>(a.ub[at(g, i, j, v)]);
+  for (int v = 0; v < 4; ++v) r[v] = a.out[at(g, i, j, v)];
   const int gx[4] = {0, 1, g.nx + 2, g.nx + 3};
   for (int k = 0; k < 4; ++k) {  // x-face ghosts of row j
     const Src s = ghost_source(g, gx[k], j);
@@ -298,7 +327,8 @@ hipError_t adjoint_step(const e2d_params& p, const DevBuffers& b, int buf, doubl
   hipLaunchKernelGGL(k_adj_faces<P>, dim3((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4), dim3(256), 0, st, a);
   hipLaunchKernelGGL(k_adj_cells<P>, dim3((g.isize - 2 + 63) / 64, (g.jsize - 2 + 3) / 4), dim3(256), 0, st, a);
   hipLaunchKernelGGL(k_adj_gather<P>, dim3((g.isize + 63) / 64, (g.jsize + 3) / 4), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(k_adj_fold<P>, dim3((g.nx + 63) / 64, (g.ny + 3) / 4), dim3(256), 0, st, a);
+  const int frame = 4 * g.nx + 4 * (g.ny > 4 ? g.ny - 4 : 0);
+  hipLaunchKernelGGL(k_adj_fold_frame<P>, dim3((frame + 255) / 256), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

@@ -25,7 +25,7 @@ struct Prec {
   using MU = MU_;
 };
 using P64 = Prec<double, double, double, RangedMath, RangedMath>;
-using P32 = Prec<float, float, float, ExactMath, ExactMath>;
+using P32 = Prec<float, float, float, RangedMath, RangedMath>;
 using PMix = Prec<float, float, double, FastMath, RangedMath>;
 
 // Derived parameters in precision T, as initHydroParameters computes them

@@ -974,6 +974,12 @@ __global__ __launch_bounds__(256) void k_pointwise(const Hydro<typename P::T> h,
       y[2 * k] = p;
       y[2 * k + 1] = c;
     } break;
+    case E2D_FN_DIV:
+      y[k] = M::div(static_cast<T>(x[2 * k]), static_cast<T>(x[2 * k + 1]));
+      break;
+    case E2D_FN_SQRT:
+      y[k] = M::sqrt(static_cast<T>(x[k]));
+      break;
     default: break;
   }
 }

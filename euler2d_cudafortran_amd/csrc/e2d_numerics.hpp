@@ -60,10 +60,32 @@ struct FastMath {
 // the range scaling (v_div_scale, ldexp by 2^256) and special-value fixup
 // (v_div_fixup, class tests) that act only outside that range.  Inside it the
 // results are the IEEE results bit for bit, except that a zero quotient is +0
-// whatever the sign of the numerator; a zero radicand returns zero.  (FP32
-// values of the solver come close enough to the float limits for the scaling
-// to matter, so the FP32 path keeps ExactMath.)
+// whatever the sign of the numerator; a zero radicand returns zero.
+//
+// The FP32 overloads do the same for binary32 (exponents within about +-90,
+// the solver's 1e-22..1e9 again): division is the reciprocal seed, one Newton
+// step and two remainder corrections of the IEEE sequence without v_div_scale
+// / v_div_fixup; square root takes the hardware estimate s (within 1 ulp) and
+// picks s - ulp, s or s + ulp from the signs of the exact residuals
+// x - (s -+ ulp) s, which decide on which side of the midpoints sqrt(x) lies.
 struct RangedMath {
+  static __device__ __forceinline__ float div(float a, float b) {
+    const float nb = -b;
+    float r = __builtin_amdgcn_rcpf(b);
+    r = __builtin_fmaf(__builtin_fmaf(nb, r, 1.0f), r, r);
+    float q = a * r;
+    q = __builtin_fmaf(__builtin_fmaf(nb, q, a), r, q);
+    return __builtin_fmaf(__builtin_fmaf(nb, q, a), r, q);
+  }
+  static __device__ __forceinline__ float sqrt(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const int bits = __builtin_bit_cast(int, s);
+    const float dn = __builtin_bit_cast(float, bits - 1);
+    const float up = __builtin_bit_cast(float, bits + 1);
+    float t = __builtin_fmaf(-dn, s, x) <= 0.0f ? dn : s;
+    t = __builtin_fmaf(-up, s, x) > 0.0f ? up : t;
+    return t;
+  }
   static __device__ __forceinline__ double div(double a, double b) {
     const double nb = -b;
     double r = __builtin_amdgcn_rcp(b);

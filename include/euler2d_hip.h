@@ -78,6 +78,8 @@ extern "C" {
 #define E2D_FN_HLLC 5    /* qleft[4], qright[4]       -> qgdnv[4], flux[4]  */
 #define E2D_FN_CMPFLX 6  /* qgdnv[4]                  -> flux[4]            */
 #define E2D_FN_EOS 7     /* rho, eint                 -> p, c               */
+#define E2D_FN_DIV 8     /* a, b                      -> a / b (the precision's flux arithmetic) */
+#define E2D_FN_SQRT 9    /* x                         -> sqrt(x)            */
 
 /*
  * Run parameters: the namelist groups &RUN/&MESH/&HYDRO/&OTHER of the

@@ -25,6 +25,9 @@
 // each step's contribution is added to lam in double.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+#include <cstring>
+
 #include "e2d_adjoint.hpp"
 #include "e2d_common.hpp"
 #include "e2d_internal.hpp"
@@ -46,6 +49,7 @@ struct AdjArgs {
   double* out;
   T *q, *xminb, *xmaxb, *yminb, *ymaxb, *qb, *dqxb, *dqyb, *ub;
   double dt;
+  int nsegs, seg_rows, seg_groups;  // k_adj_march decomposition
 };
 
 template <class X, class T>
@@ -196,6 +200,147 @@ __global__ __launch_bounds__(256) void k_adj_cells(const AdjArgs<P> a) {
   store4(a.dqyb, g, i, j, dyb);
 }
 
+// Passes 2+3 in one marching kernel (the default): each wavefront owns a strip
+// of 64 columns (60 output columns, 2 halo columns each side) and marches up
+// a segment of rows, as the forward step does.  At row r it forms the slopes,
+// trace sources and face states of row r once, solves the Riemann adjoint of
+// the y-face r-1/2, completes cell row r-1 (whose four face-state bars are
+// then known: x bars from row r-1, the lower y bar from the face r-3/2, the
+// upper one just computed), then the Riemann adjoint of the x-faces of row r;
+// the bar of the cell's right face comes from lane+1 by a shuffle.  Faces,
+// zero-bar skips and the completion order are those of k_adj_faces and
+// k_adj_cells, so qb, dqxb and dqyb are bit-identical to theirs.  Outputs are
+// the cells [1, size-2]^2.
+constexpr int kAdjWaves = 4;      // wavefronts per workgroup
+constexpr int kAdjStripOut = 60;  // output columns per wavefront
+
+template <class P>
+__global__ __launch_bounds__(64 * kAdjWaves) void k_adj_march(const AdjArgs<P> a) {
+  using T = typename P::T;
+  using U = typename P::U;
+  using M = typename P::M;
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int strip = lb / a.seg_groups;
+  const int seg = (lb % a.seg_groups) * kAdjWaves + wave;
+  if (seg >= a.nsegs) return;  // whole wavefront leaves: no barrier in this kernel
+
+  const Grid& g = a.g;
+  const int ci = 1 + strip * kAdjStripOut - 2 + lane;  // column of this lane
+  const int cl = ci < 0 ? 0 : (ci > g.isize - 1 ? g.isize - 1 : ci);
+  const bool out_col = lane >= 2 && lane < 62 && ci <= g.isize - 2;
+  const bool xface = lane >= 2 && lane <= 62 && ci >= 2 && ci <= g.nx + 2;  // face ci-1/2
+  const bool yface = lane >= 2 && lane < 62 && ci >= 2 && ci <= g.nx + 1;
+  const int ra = 1 + seg * a.seg_rows;
+  const int rb = min(ra + a.seg_rows, g.jsize - 1);  // output rows [ra, rb)
+
+  U dtdxU, dtdyU;
+  step_ratios(a, dtdxU, dtdyU);
+  const T dtdx = static_cast<T>(dtdxU), dtdy = static_cast<T>(dtdyU);
+  const T zero4[4] = {T(0), T(0), T(0), T(0)};
+  auto row = [&](int r) { return r < 0 ? 0 : (r > g.jsize - 1 ? g.jsize - 1 : r); };
+
+  T qa[4], qb[4], qc[4];
+  load4(a.q, g, cl, row(ra - 2), qa);
+  load4(a.q, g, cl, row(ra - 1), qb);
+  // row r-1, carried: q, slopes, trace sources and the bars of its xmin, xmax, ymin face states
+  T pq[4], pdx[4], pdy[4], ps[4], pbxm[4], pbxp[4], pbym[4];
+
+  for (int r = ra - 1; r <= rb; ++r) {
+    load4(a.q, g, cl, row(r + 1), qc);
+    T qxm[4], qxp[4], dqx[4], dqy[4], s[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      qxm[v] = from_left(qb[v]);
+      qxp[v] = from_right(qb[v]);
+    }
+    cell_slopes(a.h, qb, qxp, qxm, qc, qa, dqx, dqy);
+    trace_sources<T, M>(a.h, qb, dqx, dqy, s);
+
+    T bym[4], bxm[4], bxp[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) bym[v] = bxm[v] = bxp[v] = T(0);
+    if (r >= ra) {
+      // y-face between rows r-1 and r
+      T ylb[4] = {T(0), T(0), T(0), T(0)};
+      if (yface && r >= 2 && r <= g.ny + 2) {
+        T fb[4];
+        bool any = false;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          double x = 0.0;
+          if (interior(g, ci, r)) x += a.lam[at(g, ci, r, v)];
+          if (interior(g, ci, r - 1)) x -= a.lam[at(g, ci, r - 1, v)];
+          fb[v] = static_cast<T>(x * static_cast<double>(dtdyU));
+          any |= (x != 0.0);
+        }
+        if (any) {
+          T fl[4], fr[4];
+          face_state(a.h, pq, pdy, ps, dtdy, true, fl);
+          face_state(a.h, qb, dqy, s, dtdy, false, fr);
+          riemann_y_adj<M>(a.h, fl, fr, fb, ylb, bym);
+        }
+      }
+      // cell row r-1 is complete
+      if (r - 1 >= ra && out_col) {
+        T qbar[4] = {T(0), T(0), T(0), T(0)}, dxb[4] = {T(0), T(0), T(0), T(0)}, dyb[4] = {T(0), T(0), T(0), T(0)};
+        bool any = false;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          any |= (pbxm[v] != T(0)) | (pbxp[v] != T(0)) | (pbym[v] != T(0)) | (ylb[v] != T(0));
+        if (any) {
+          T sb[4] = {T(0), T(0), T(0), T(0)};
+          face_state_adj(a.h, pq, pdx, ps, dtdx, false, pbxm, qbar, dxb, sb);
+          face_state_adj(a.h, pq, pdx, ps, dtdx, true, pbxp, qbar, dxb, sb);
+          face_state_adj(a.h, pq, pdy, ps, dtdy, false, pbym, qbar, dyb, sb);
+          face_state_adj(a.h, pq, pdy, ps, dtdy, true, ylb, qbar, dyb, sb);
+          trace_sources_adj<M>(a.h, pq, pdx, pdy, sb, qbar, dxb, dyb);
+        }
+        store4(a.qb, g, ci, r - 1, qbar);
+        store4(a.dqxb, g, ci, r - 1, dxb);
+        store4(a.dqyb, g, ci, r - 1, dyb);
+      }
+      // x-faces of row r
+      if (r < rb) {
+        T xmax_own[4], xl[4], xr[4], xlb[4] = {T(0), T(0), T(0), T(0)};
+        face_state(a.h, qb, dqx, s, dtdx, true, xmax_own);
+        face_state(a.h, qb, dqx, s, dtdx, false, xr);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) xl[v] = from_left(xmax_own[v]);
+        if (xface && r >= 2 && r <= g.ny + 1) {
+          T fb[4];
+          bool any = false;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            double x = 0.0;
+            if (interior(g, ci, r)) x += a.lam[at(g, ci, r, v)];
+            if (interior(g, ci - 1, r)) x -= a.lam[at(g, ci - 1, r, v)];
+            fb[v] = static_cast<T>(x * static_cast<double>(dtdxU));
+            any |= (x != 0.0);
+          }
+          if (any) riemann_two_shock_adj<M>(a.h, xl, xr, fb, xlb, bxm);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) bxp[v] = from_right(xlb[v]);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      pq[v] = qb[v];
+      pdx[v] = dqx[v];
+      pdy[v] = dqy[v];
+      ps[v] = s[v];
+      pbxm[v] = bxm[v];
+      pbxp[v] = bxp[v];
+      pbym[v] = bym[v];
+      qa[v] = qb[v];
+      qb[v] = qc[v];
+    }
+  }
+  (void)zero4;
+}
+
 // all cells: bar of q from the cell's own trace and the limiters of the cell
 // and its four neighbours, then bar of u
 template <class P>
@@ -299,6 +444,36 @@ __global__ __launch_bounds__(256) void k_adj_fold_frame(const AdjArgs<P> a) {
   for (int v = 0; v < 4; ++v) a.out[at(g, i, j, v)] = r[v];
 }
 
+// E2D_ADJOINT_PASSES=separate runs passes 2 and 3 as the two gather kernels
+// k_adj_faces / k_adj_cells (face bars through HBM) instead of k_adj_march;
+// both give the same bits.
+bool adjoint_passes_separate() {
+  const char* env = getenv("E2D_ADJOINT_PASSES");
+  return env && strcmp(env, "separate") == 0;
+}
+
+// Rows per k_adj_march segment (E2D_ADJOINT_SEG_ROWS overrides): the rule of
+// the forward step, about rows/170 x 3 / resident waves per SIMD.
+template <class P>
+int march_seg_rows(int rows) {
+  if (const char* env = getenv("E2D_ADJOINT_SEG_ROWS")) {
+    const int r = atoi(env);
+    if (r >= 1) return r;
+  }
+  static int waves_per_simd = 0;
+  if (waves_per_simd == 0) {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(k_adj_march<P>),
+                                                     64 * kAdjWaves, 0) != hipSuccess ||
+        blocks <= 0)
+      blocks = 2;
+    waves_per_simd = blocks * kAdjWaves / 4;
+    if (waves_per_simd < 1) waves_per_simd = 1;
+  }
+  const int r = static_cast<int>(rows / 170.0 * 3.0 / waves_per_simd + 0.5);
+  return r < 8 ? 8 : (r > 32 ? 32 : r);
+}
+
 template <class P>
 hipError_t adjoint_step(const e2d_params& p, const DevBuffers& b, int buf, double dt, const double* lam_in,
                         double* lam_out, const AdjointScratch& s, hipStream_t st) {
@@ -324,8 +499,17 @@ hipError_t adjoint_step(const e2d_params& p, const DevBuffers& b, int buf, doubl
   if (e != hipSuccess) return e;
   const Grid& g = a.g;
   hipLaunchKernelGGL(k_adj_prim<P>, dim3((g.isize + 63) / 64, (g.jsize + 3) / 4), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(k_adj_faces<P>, dim3((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(k_adj_cells<P>, dim3((g.isize - 2 + 63) / 64, (g.jsize - 2 + 3) / 4), dim3(256), 0, st, a);
+  if (adjoint_passes_separate()) {
+    hipLaunchKernelGGL(k_adj_faces<P>, dim3((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_adj_cells<P>, dim3((g.isize - 2 + 63) / 64, (g.jsize - 2 + 3) / 4), dim3(256), 0, st,
+                       a);
+  } else {
+    a.seg_rows = march_seg_rows<P>(g.jsize - 2);
+    a.nsegs = (g.jsize - 2 + a.seg_rows - 1) / a.seg_rows;
+    a.seg_groups = (a.nsegs + kAdjWaves - 1) / kAdjWaves;
+    const int nstrips = (g.isize - 2 + kAdjStripOut - 1) / kAdjStripOut;
+    hipLaunchKernelGGL(k_adj_march<P>, dim3(nstrips * a.seg_groups), dim3(64 * kAdjWaves), 0, st, a);
+  }
   hipLaunchKernelGGL(k_adj_gather<P>, dim3((g.isize + 63) / 64, (g.jsize + 3) / 4), dim3(256), 0, st, a);
   const int frame = 4 * g.nx + 4 * (g.ny > 4 ? g.ny - 4 : 0);
   hipLaunchKernelGGL(k_adj_fold_frame<P>, dim3((frame + 255) / 256), dim3(256), 0, st, a);

@@ -374,3 +374,27 @@ def test_fp32_rollout_against_fp64(gpu_lib):
         err = np.linalg.norm(grads[prec] - ref) / np.linalg.norm(ref)
         print(f"precision {prec}: rollout gradient vs fp64, relative L2 {err:.2e}")
         assert err <= 1e-2
+
+
+@pytest.mark.parametrize("prec", [e2d.PREC_FP64, e2d.PREC_FP32, e2d.PREC_MIXED])
+def test_marching_adjoint_equals_separate_passes(gpu_lib, prec, monkeypatch):
+    """The marching adjoint kernel (face bars in registers) against the two
+    gather passes (face bars through HBM, E2D_ADJOINT_PASSES=separate): the
+    same bits, on odd sizes, thin grids and every boundary type."""
+    for bc, (nx, ny) in [((1, 1, 1, 1), (96, 80)), ((3, 3, 3, 3), (61, 37)), ((1, 3, 2, 1), (130, 7)),
+                         ((2, 2, 2, 2), (4, 4)), ((1, 1, 3, 3), (64, 200))]:
+        p = _params(nx, ny, precision=prec, boundary_type_xmin=bc[0], boundary_type_xmax=bc[1],
+                    boundary_type_ymin=bc[2], boundary_type_ymax=bc[3])
+        lam = np.random.default_rng(nx * ny).normal(size=(nx + 4, ny + 4, 4))
+        res = {}
+        for mode in ("separate", "march"):
+            monkeypatch.setenv("E2D_ADJOINT_PASSES", mode)
+            with e2d.HydroRun(p) as run:
+                U = _evolved_diag(run, 25)
+                run.upload(0, U)
+                dt = run.compute_dt(0)
+                a, b = run.field_alloc(), run.field_alloc()
+                run.field_upload(a, lam)
+                run.adjoint_step(0, dt, a, b)
+                res[mode] = run.field_download(b)
+        assert np.array_equal(res["separate"], res["march"]), f"bc={bc} {nx}x{ny}"

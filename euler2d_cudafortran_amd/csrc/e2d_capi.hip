@@ -547,6 +547,11 @@ struct Rollout {
   double* lam;
   std::vector<int> free_slots;
   int64_t nfwd = 0;
+  // step whose state each buffer holds (-1: none of this rollout).  A step
+  // leaves its input in place (only ghost cells, which every consumer
+  // rebuilds from the interior, are rewritten), so after advancing to k the
+  // other buffer still holds k-1 and a restore of a held state is skipped.
+  int held[2] = {-1, -1};
 
   int buf_of(int k) const { return (n0 + k) % 2; }
   int save(int slot, int k) {
@@ -555,13 +560,16 @@ struct Rollout {
     return E2D_OK;
   }
   int restore(int slot, int k) {
+    if (held[buf_of(k)] == k) return E2D_OK;
     HIP_TRY(hipMemcpyAsync(c->b.u[buf_of(k)], c->snapshots[slot], state_bytes(c), hipMemcpyDeviceToDevice,
                            c->stream));
+    held[buf_of(k)] = k;
     return E2D_OK;
   }
   int advance(int from, int to) {
     for (int k = from; k < to; ++k) {
       if (int rc = forward_step(c, n0 + k, dts[k])) return rc;
+      held[buf_of(k + 1)] = k + 1;
       ++nfwd;
     }
     return E2D_OK;
@@ -640,6 +648,7 @@ int e2d_adjoint_rollout(e2d_ctx* c, int nStep0, int nsteps, const double* dts, i
   if (lam_0 != lam_n)
     HIP_TRY(hipMemcpyAsync(lam_0, lam_n, field_bytes(c), hipMemcpyDeviceToDevice, c->stream));
   Rollout r{c, nStep0, dts, lam_0, {}, 0};
+  r.held[r.buf_of(0)] = 0;
   for (int k = nsnap - 1; k >= 1; --k) r.free_slots.push_back(k);
   if (int rc = r.save(0, 0)) return rc;
   if (int rc = r.reverse(0, nsteps, 0)) return rc;

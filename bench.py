@@ -184,8 +184,8 @@ def _rollout(e2d, nx, prec, dts, nsnap, lam_of):
 
 def adjoint_metrics(e2d):
     """Adjoint cost and accuracy.
-    * one adjoint step against one forward step at 2048^2 FP64 (kernel time
-      from hipEvents);
+    * one adjoint step (dense seeded lam) against one forward step at 2048^2
+      FP64 (kernel time from hipEvents, state after 100 steps);
     * 512^2 x 400 steps, 20 snapshots, J = sum of rho over [0.6,0.8]^2 (FP64);
     * 2048^2 x 2000 steps, 45 snapshots, J = sum over the interior of
       (rho_T - rho_target)^2 with a seeded smooth target: FP64 and FP32
@@ -200,6 +200,9 @@ def adjoint_metrics(e2d):
     run.synchronize()
     buf, dt = run.current(), run.get_dt()
     a, b = run.field_alloc(), run.field_alloc()
+    # a dense seeded lam: every face carries a nonzero flux bar (a zero bar
+    # skips that face's Riemann adjoint)
+    run.field_upload(a, np.random.default_rng(7).normal(size=(2052, 2052, 4)))
     run.enable_timers(True)
     run.get_timers()
     for _ in range(20):

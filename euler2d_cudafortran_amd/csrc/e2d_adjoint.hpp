@@ -108,13 +108,14 @@ E2D_HDI void cons_to_prim_adj(const Hydro<T>& h, const T u[4], const T qb[4], T 
   rho_b += pa_b * gm1 * eint + pb_b * h.smallp;
   const T eint_b = pa_b * gm1 * rho;
   const T e0_b = eint_b, kin_b = -eint_b;
-  ub[1] += e0_b / rho;
-  rho_b -= e0_b * e0 / rho;
+  const T irho = M::div(T(1), rho);
+  ub[1] += e0_b * irho;
+  rho_b -= e0_b * e0 * irho;
   vx_b += kin_b * vx;
   vy_b += kin_b * vy;
-  ub[2] += vx_b / rho;
-  ub[3] += vy_b / rho;
-  rho_b -= (vx_b * vx + vy_b * vy) / rho;
+  ub[2] += vx_b * irho;
+  ub[3] += vy_b * irho;
+  rho_b -= (vx_b * vx + vy_b * vy) * irho;
   if (u[0] > h.smallr) ub[0] += rho_b;
 }
 
@@ -162,12 +163,13 @@ E2D_HDI void trace_sources_adj(const Hydro<T>& h, const T q[4], const T dqx[4],
     dqxb[k] += -u * sb[k];
     dqyb[k] += -v * sb[k];
   }
-  dqxb[1] += -sb[2] / r;
-  dqyb[1] += -sb[3] / r;
+  const T ir = M::div(T(1), r);
+  dqxb[1] += -sb[2] * ir;
+  dqyb[1] += -sb[3] * ir;
   const T dv_b = -sb[0] * r - sb[1] * h.gamma0 * p;
   dqxb[2] += dv_b;
   dqyb[3] += dv_b;
-  qb[0] += -sb[0] * dv + (sb[2] * dqx[1] + sb[3] * dqy[1]) / (r * r);
+  qb[0] += -sb[0] * dv + (sb[2] * dqx[1] + sb[3] * dqy[1]) * (ir * ir);
   qb[1] += -sb[1] * dv * h.gamma0;
 }
 
@@ -255,32 +257,35 @@ E2D_HDI T newton_step_adj(const Hydro<T>& h, T pk, T cl, T cr, T pl, T pr,
     pk_b -= delp_b;
   const T m_b = t_b * (usl - usr);
   const T usl_b = t_b * m, usr_b = -t_b * m;
-  const T pm_b = m_b / S, s_b = -m_b * m / S;
+  const T iS = M::div(T(1), S), iwwl = M::div(T(1), wwl), iwwr = M::div(T(1), wwr);
+  const T idzl = M::div(T(1), dzl), idzr = M::div(T(1), dzr);
+  const T ipl = M::div(T(1), pl), ipr = M::div(T(1), pr);
+  const T pm_b = m_b * iS, s_b = -m_b * m * iS;
   const T zr_b = pm_b * zl + s_b, zl_b = pm_b * zr + s_b;
   // usl = ul - (pk - pl)/wwl ; usr = ur + (pk - pr)/wwr
   ul_b += usl_b;
   const T c1_b = -usl_b;
-  pk_b += c1_b / wwl;
-  pl_b -= c1_b / wwl;
-  T wwl_b = -c1_b * c1 / wwl;
+  pk_b += c1_b * iwwl;
+  pl_b -= c1_b * iwwl;
+  T wwl_b = -c1_b * c1 * iwwl;
   ur_b += usr_b;
   const T c2_b = usr_b;
-  pk_b += c2_b / wwr;
-  pr_b -= c2_b / wwr;
-  T wwr_b = -c2_b * c2 / wwr;
+  pk_b += c2_b * iwwr;
+  pr_b -= c2_b * iwwr;
+  T wwr_b = -c2_b * c2 * iwwr;
   // z = 2 ww^3 / (ww^2 + c)
-  wwl_b += zl_b / dzl * T(6.0) * wwl * wwl - zl_b * zl / dzl * T(2.0) * wwl;
-  cl_b += -zl_b * zl / dzl;
-  wwr_b += zr_b / dzr * T(6.0) * wwr * wwr - zr_b * zr / dzr * T(2.0) * wwr;
-  cr_b += -zr_b * zr / dzr;
+  wwl_b += zl_b * idzl * T(6.0) * wwl * wwl - zl_b * zl * idzl * T(2.0) * wwl;
+  cl_b += -zl_b * zl * idzl;
+  wwr_b += zr_b * idzr * T(6.0) * wwr * wwr - zr_b * zr * idzr * T(2.0) * wwr;
+  cr_b += -zr_b * zr * idzr;
   // ww = sqrt(c (1 + g6 (pk - p0)/p0))
-  const T al_b = wwl_b * T(0.5) / wwl, ar_b = wwr_b * T(0.5) / wwr;
+  const T al_b = wwl_b * T(0.5) * iwwl, ar_b = wwr_b * T(0.5) * iwwr;
   cl_b += al_b * (T(1.0) + xl);
   cr_b += ar_b * (T(1.0) + xr);
   const T xl_b = al_b * cl, xr_b = ar_b * cr;
-  pk_b += xl_b * g6 / pl + xr_b * g6 / pr;
-  pl_b += -xl_b * g6 * pk / (pl * pl);
-  pr_b += -xr_b * g6 * pk / (pr * pr);
+  pk_b += xl_b * g6 * ipl + xr_b * g6 * ipr;
+  pl_b += -xl_b * g6 * pk * (ipl * ipl);
+  pr_b += -xr_b * g6 * pk * (ipr * ipr);
   return pk_b;
 }
 
@@ -362,7 +367,9 @@ E2D_HDI void riemann_two_shock_adj(const Hydro<T>& h, const T ql[4], const T qr[
   }
   gs[3] = right ? ql[3] : qr[3];
 
-  // ---- reverse
+  // ---- reverse (derivative arithmetic with one reciprocal per denominator)
+  const T iro = M::div(T(1), ro), irs = M::div(T(1), rs);
+  const T ipl = M::div(T(1), pl), ipr = M::div(T(1), pr);
   T gb[4] = {T(0.0), T(0.0), T(0.0), T(0.0)};
   godunov_flux_adj(h, gs, fb, gb);
   if (right)
@@ -387,8 +394,9 @@ E2D_HDI void riemann_two_shock_adj(const Hydro<T>& h, const T ql[4], const T qr[
     ps_b += gb[1] * frac;
     po_b += gb[1] * (T(1.0) - frac);
     // frac = 0.5 (1 + sum/scr)
-    T sum_b = frac_b * T(0.5) / scr;
-    const T scr_b = -frac_b * T(0.5) * sum / (scr * scr);
+    const T iscr = M::div(T(1), scr);
+    T sum_b = frac_b * T(0.5) * iscr;
+    const T scr_b = -frac_b * T(0.5) * sum * (iscr * iscr);
     T dif_b = T(0.0);
     if (take_dif)
       dif_b = scr_b;
@@ -398,8 +406,8 @@ E2D_HDI void riemann_two_shock_adj(const Hydro<T>& h, const T ql[4], const T qr[
     T co_b = T(0.0), cs_b = T(0.0), s_uo_b = T(0.0);
     if (shock) {
       const T ush_b = spout_b + spin_b;
-      wo_b += ush_b / ro;
-      ro_b -= ush_b * wo / (ro * ro);
+      wo_b += ush_b * iro;
+      ro_b -= ush_b * wo * (iro * iro);
       s_uo_b -= ush_b;
     } else {
       co_b += spout_b;
@@ -410,27 +418,28 @@ E2D_HDI void riemann_two_shock_adj(const Hydro<T>& h, const T ql[4], const T qr[
     uo_b += s_uo_b * sg;
     // cs = max(smallc, sqrt|g ps / rs|)
     if (!(h.smallc > cs_r) && cs_r > T(0.0)) {
-      const T X_b = cs_b * T(0.5) / cs_r * sgn(X);
-      ps_b += X_b * g / rs;
-      rs_b -= X_b * g * ps / (rs * rs);
+      const T X_b = cs_b * T(0.5) * M::div(T(1), cs_r) * sgn(X);
+      ps_b += X_b * g * irs;
+      rs_b -= X_b * g * ps * (irs * irs);
     }
     // co = max(smallc, sqrt|g po / ro|)
     if (!(h.smallc > co_r) && co_r > T(0.0)) {
-      const T Y_b = co_b * T(0.5) / co_r * sgn(Y);
-      po_b += Y_b * g / ro;
-      ro_b -= Y_b * g * po / (ro * ro);
+      const T Y_b = co_b * T(0.5) * M::div(T(1), co_r) * sgn(Y);
+      po_b += Y_b * g * iro;
+      ro_b -= Y_b * g * po * (iro * iro);
     }
   }
   // rs = max(ro / (1 + ro (po - ps) / wo^2), smallr)
   if (rs_r > h.smallr) {
     const T rsr_b = rs_b;
-    ro_b += rsr_b / D;
-    const T E_b = -rsr_b * ro / (D * D);
-    const T w2 = wo * wo;
-    ro_b += E_b * (po - ps) / w2;
-    po_b += E_b * ro / w2;
-    ps_b -= E_b * ro / w2;
-    wo_b -= E_b * T(2.0) * E / wo;
+    const T iD = M::div(T(1), D);
+    ro_b += rsr_b * iD;
+    const T E_b = -rsr_b * ro * (iD * iD);
+    const T iw2 = M::div(T(1), wo * wo);
+    ro_b += E_b * (po - ps) * iw2;
+    po_b += E_b * ro * iw2;
+    ps_b -= E_b * ro * iw2;
+    wo_b -= E_b * T(2.0) * E * M::div(T(1), wo);
   }
   T rl_b = T(0.0), ul_b = T(0.0), pl_b = T(0.0), wl_b = T(0.0);
   T rr_b = T(0.0), ur_b = T(0.0), pr_b = T(0.0), wr_b = T(0.0);
@@ -449,27 +458,28 @@ E2D_HDI void riemann_two_shock_adj(const Hydro<T>& h, const T ql[4], const T qr[
   ul_b += T(0.5) * us_b;
   ur_b += T(0.5) * us_b;
   const T a_b = T(0.5) * us_b, b_b = -T(0.5) * us_b;
-  pl_b += a_b / wl;
-  ps_b -= a_b / wl;
-  wl_b -= a_b * a_us / wl;
-  pr_b += b_b / wr;
-  ps_b -= b_b / wr;
-  wr_b -= b_b * b_us / wr;
+  const T iwl = M::div(T(1), wl), iwr = M::div(T(1), wr);
+  pl_b += a_b * iwl;
+  ps_b -= a_b * iwl;
+  wl_b -= a_b * a_us * iwl;
+  pr_b += b_b * iwr;
+  ps_b -= b_b * iwr;
+  wr_b -= b_b * b_us * iwr;
   // wl = sqrt(cl (1 + g6 (ps - pl)/pl))
   T cl_b = T(0.0), cr_b = T(0.0);
   {
-    const T W_b = wl_b * T(0.5) / wl;
+    const T W_b = wl_b * T(0.5) * iwl;
     cl_b += W_b * (T(1.0) + xlf);
     const T x_b = W_b * cl;
-    ps_b += x_b * g6 / pl;
-    pl_b += -x_b * g6 * ps / (pl * pl);
+    ps_b += x_b * g6 * ipl;
+    pl_b += -x_b * g6 * ps * (ipl * ipl);
   }
   {
-    const T W_b = wr_b * T(0.5) / wr;
+    const T W_b = wr_b * T(0.5) * iwr;
     cr_b += W_b * (T(1.0) + xrf);
     const T x_b = W_b * cr;
-    ps_b += x_b * g6 / pr;
-    pr_b += -x_b * g6 * ps / (pr * pr);
+    ps_b += x_b * g6 * ipr;
+    pr_b += -x_b * g6 * ps * (ipr * ipr);
   }
   // Newton iterations, last to first (p_k recomputed from p_0)
   T p_b = ps_b;
@@ -482,8 +492,9 @@ E2D_HDI void riemann_two_shock_adj(const Hydro<T>& h, const T ql[4], const T qr[
   // p0 = max(num/den, 0)
   T wl0_b = T(0.0), wr0_b = T(0.0);
   if (p0r > T(0.0)) {
-    const T num_b = p_b / den;
-    const T den_b = -p_b * p0r / den;
+    const T iden = M::div(T(1), den);
+    const T num_b = p_b * iden;
+    const T den_b = -p_b * p0r * iden;
     wr0_b += num_b * pl + num_b * wl0 * (ul - ur) + den_b;
     pl_b += num_b * wr0;
     wl0_b += num_b * pr + num_b * wr0 * (ul - ur) + den_b;
@@ -491,8 +502,8 @@ E2D_HDI void riemann_two_shock_adj(const Hydro<T>& h, const T ql[4], const T qr[
     ul_b += num_b * wl0 * wr0;
     ur_b -= num_b * wl0 * wr0;
   }
-  cl_b += wl0_b * T(0.5) / wl0;
-  cr_b += wr0_b * T(0.5) / wr0;
+  cl_b += wl0_b * T(0.5) * M::div(T(1), wl0);
+  cr_b += wr0_b * T(0.5) * M::div(T(1), wr0);
   // c = g p r
   pl_b += cl_b * g * rl;
   rl_b += cl_b * g * pl;

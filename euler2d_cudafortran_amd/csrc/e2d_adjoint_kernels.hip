@@ -211,11 +211,22 @@ __global__ __launch_bounds__(256) void k_adj_cells(const AdjArgs<P> a) {
 // zero-bar skips and the completion order are those of k_adj_faces and
 // k_adj_cells, so qb, dqxb and dqyb are bit-identical to theirs.  Outputs are
 // the cells [1, size-2]^2.
+// At least 2 resident wavefronts per SIMD: left to itself the compiler gives
+// the FP64 kernel more than 256 VGPRs (1 wavefront per SIMD); the cap costs a
+// 12-byte spill and is 1.36x faster (2048^2, dense lam).
+#ifndef E2D_ADJ_MARCH_MIN_WAVES
+#define E2D_ADJ_MARCH_MIN_WAVES 2
+#endif
+#if E2D_ADJ_MARCH_MIN_WAVES > 0
+#define E2D_ADJ_MARCH_ATTR __attribute__((amdgpu_waves_per_eu(E2D_ADJ_MARCH_MIN_WAVES)))
+#else
+#define E2D_ADJ_MARCH_ATTR
+#endif
 constexpr int kAdjWaves = 4;      // wavefronts per workgroup
 constexpr int kAdjStripOut = 60;  // output columns per wavefront
 
 template <class P>
-__global__ __launch_bounds__(64 * kAdjWaves) void k_adj_march(const AdjArgs<P> a) {
+__global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march(const AdjArgs<P> a) {
   using T = typename P::T;
   using U = typename P::U;
   using M = typename P::M;

@@ -49,7 +49,8 @@ struct AdjArgs {
   double* out;
   T *q, *xminb, *xmaxb, *yminb, *ymaxb, *qb, *dqxb, *dqyb, *ub;
   double dt;
-  int nsegs, seg_rows, seg_groups;  // k_adj_march decomposition
+  int nsegs, seg_rows, seg_groups;     // k_adj_march decomposition
+  int gnsegs, gseg_rows, gseg_groups;  // k_adj_gather_march decomposition
 };
 
 template <class X, class T>
@@ -62,6 +63,15 @@ template <class T>
 __device__ __forceinline__ void store4(T* a, const Grid& g, int i, int j, const T x[4]) {
 #pragma unroll
   for (int v = 0; v < 4; ++v) a[at(g, i, j, v)] = x[v];
+}
+
+// primitives of cell (i, j) from the (boundary-filled) state, exactly as
+// k_adj_prim stores them
+template <class P>
+__device__ __forceinline__ void prim_at(const AdjArgs<P>& a, int i, int j, typename P::T q[4]) {
+  typename P::T u[4];
+  load4(a.u, a.g, i, j, u);
+  cons_to_prim<typename P::T, typename P::M>(a.h, u, q);
 }
 
 // dt/dx and dt/dy as the forward step forms them (in the update type U)
@@ -253,13 +263,13 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
   auto row = [&](int r) { return r < 0 ? 0 : (r > g.jsize - 1 ? g.jsize - 1 : r); };
 
   T qa[4], qb[4], qc[4];
-  load4(a.q, g, cl, row(ra - 2), qa);
-  load4(a.q, g, cl, row(ra - 1), qb);
+  prim_at(a, cl, row(ra - 2), qa);
+  prim_at(a, cl, row(ra - 1), qb);
   // row r-1, carried: q, slopes, trace sources and the bars of its xmin, xmax, ymin face states
   T pq[4], pdx[4], pdy[4], ps[4], pbxm[4], pbxp[4], pbym[4];
 
   for (int r = ra - 1; r <= rb; ++r) {
-    load4(a.q, g, cl, row(r + 1), qc);
+    prim_at(a, cl, row(r + 1), qc);
     T qxm[4], qxp[4], dqx[4], dqy[4], s[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -350,6 +360,127 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
     }
   }
   (void)zero4;
+}
+
+// Pass 4 as a marching kernel (with k_adj_march): wavefronts own strips of 64
+// columns (60 output columns) and march up segments of rows.  Each cell's x-
+// and y-limiter adjoints are formed once, from primitives computed from u in
+// rolling rows; a cell collects its neighbours' parts by shuffles (x) and from
+// the rows before and after (y), in k_adj_gather's order and with its
+// zero-bar skips, then applies the primitive adjoint.  Outputs are all cells.
+constexpr int kGatherStripOut = 60;
+
+template <class P>
+__global__ __launch_bounds__(64 * kAdjWaves) void k_adj_gather_march(const AdjArgs<P> a) {
+  using T = typename P::T;
+  using M = typename P::M;
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int strip = lb / a.gseg_groups;
+  const int seg = (lb % a.gseg_groups) * kAdjWaves + wave;
+  if (seg >= a.gnsegs) return;  // whole wavefront leaves: no barrier in this kernel
+
+  const Grid& g = a.g;
+  const int ci = strip * kGatherStripOut + lane - 2;  // column of this lane
+  const int cl = ci < 0 ? 0 : (ci > g.isize - 1 ? g.isize - 1 : ci);
+  const bool out_col = lane >= 2 && lane < 62 && ci < g.isize;
+  const bool slope_col = ci >= 1 && ci <= g.isize - 2;  // the lane's cell has slopes
+  const int ra = seg * a.gseg_rows;
+  const int rb = min(ra + a.gseg_rows, g.jsize);  // output rows [ra, rb)
+  const bool limited = slopes_limited(a.h);
+  const T st = static_cast<T>(a.h.slope_type);
+  auto row = [&](int r) { return r < 0 ? 0 : (r > g.jsize - 1 ? g.jsize - 1 : r); };
+
+  T qm[4], q0[4], qp[4];  // primitives of rows r-1, r, r+1
+  prim_at(a, cl, row(ra - 2), qm);
+  prim_at(a, cl, row(ra - 1), q0);
+  // y-limiter parts of earlier rows: plus-side part of rows r-2 and r-1, own part of row r-1
+  T yp2[4], yp1[4], y01[4];
+  bool vp2[4], vp1[4], v01[4];
+  T qbx1[4];  // bar of q of row r-1 with its own and x-limiter parts
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    yp2[v] = yp1[v] = y01[v] = qbx1[v] = T(0);
+    vp2[v] = vp1[v] = v01[v] = false;
+  }
+
+  for (int r = ra - 1; r <= rb; ++r) {
+    prim_at(a, cl, row(r + 1), qp);
+    const bool cell_has = slope_col && r >= 1 && r <= g.jsize - 2;
+    // y-limiter adjoint of cell (ci, r)
+    T ym[4], y0[4], yp[4];
+    bool yv[4];
+    T dyb[4] = {T(0), T(0), T(0), T(0)}, dxb[4] = {T(0), T(0), T(0), T(0)};
+    if (cell_has) {
+      load4(a.dqyb, g, ci, r, dyb);
+      load4(a.dqxb, g, ci, r, dxb);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      ym[v] = y0[v] = yp[v] = T(0);
+      yv[v] = limited && cell_has && dyb[v] != T(0);
+      if (yv[v]) limited_slope_adj(st, qm[v], q0[v], qp[v], dyb[v], ym[v], y0[v], yp[v]);
+    }
+    // x-limiter adjoint of cell (ci, r) and the x parts of its bar of q
+    T qbx[4] = {T(0), T(0), T(0), T(0)};
+    if (r >= ra && r < rb) {
+      T xm[4], x0[4], xp[4];
+      bool xv[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        xm[v] = x0[v] = xp[v] = T(0);
+        xv[v] = limited && cell_has && dxb[v] != T(0);
+        const T ql = from_left(q0[v]), qr = from_right(q0[v]);
+        if (xv[v]) limited_slope_adj(st, ql, q0[v], qr, dxb[v], xm[v], x0[v], xp[v]);
+      }
+      if (cell_has) load4(a.qb, g, ci, r, qbx);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const T lp = from_left(xp[v]), rm = from_right(xm[v]);
+        const bool lv = __shfl_up(static_cast<int>(xv[v]), 1, 64) != 0;
+        const bool rv = __shfl_down(static_cast<int>(xv[v]), 1, 64) != 0;
+        if (limited) {
+          if (lv) qbx[v] += lp;  // left neighbour: this cell is its plus side
+          if (xv[v]) qbx[v] += x0[v];
+          if (rv) qbx[v] += rm;  // right neighbour: this cell is its minus side
+        }
+      }
+    }
+    // cell row r-1 is complete
+    if (r - 1 >= ra && r - 1 < rb && out_col) {
+      const int jr = r - 1;
+      T qb[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        qb[v] = qbx1[v];
+        if (vp2[v]) qb[v] += yp2[v];
+        if (v01[v]) qb[v] += y01[v];
+        if (yv[v]) qb[v] += ym[v];
+      }
+      T u[4], ub[4] = {T(0), T(0), T(0), T(0)};
+      load4(a.u, g, ci, jr, u);
+      cons_to_prim_adj<M>(a.h, u, qb, ub);
+      if (interior(g, ci, jr)) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) a.out[at(g, ci, jr, v)] = a.lam[at(g, ci, jr, v)] + static_cast<double>(ub[v]);
+      } else {
+        store4(a.ub, g, ci, jr, ub);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      yp2[v] = yp1[v];
+      vp2[v] = vp1[v];
+      yp1[v] = yp[v];
+      vp1[v] = yv[v];
+      y01[v] = y0[v];
+      v01[v] = yv[v];
+      qbx1[v] = qbx[v];
+      qm[v] = q0[v];
+      q0[v] = qp[v];
+    }
+  }
 }
 
 // all cells: bar of q from the cell's own trace and the limiters of the cell
@@ -463,26 +594,28 @@ bool adjoint_passes_separate() {
   return env && strcmp(env, "separate") == 0;
 }
 
-// Rows per k_adj_march segment (E2D_ADJOINT_SEG_ROWS overrides): the rule of
-// the forward step, about rows/170 x 3 / resident waves per SIMD.
-template <class P>
+// Rows per k_adj_march segment (E2D_ADJOINT_SEG_ROWS overrides): about
+// rows/340, clamped to [4, 16].  The kernel holds 2 wavefronts per SIMD, so
+// short segments (many wavefronts, several rounds) balance the data-dependent
+// Riemann work best: measured at 2048^2 FP64 with a dense lam, 4-7 rows give
+// 860-880 us per adjoint step, 12 rows 896 us, 18 rows 950 us, 24 rows 1030 us.
 int march_seg_rows(int rows) {
   if (const char* env = getenv("E2D_ADJOINT_SEG_ROWS")) {
     const int r = atoi(env);
     if (r >= 1) return r;
   }
-  static int waves_per_simd = 0;
-  if (waves_per_simd == 0) {
-    int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(k_adj_march<P>),
-                                                     64 * kAdjWaves, 0) != hipSuccess ||
-        blocks <= 0)
-      blocks = 2;
-    waves_per_simd = blocks * kAdjWaves / 4;
-    if (waves_per_simd < 1) waves_per_simd = 1;
+  const int r = static_cast<int>(rows / 340.0 + 0.5);
+  return r < 4 ? 4 : (r > 16 ? 16 : r);
+}
+
+// Rows per k_adj_gather_march segment (E2D_ADJOINT_GATHER_ROWS overrides):
+// short, so that the light, memory-bound kernel has many wavefronts in flight.
+int gather_seg_rows() {
+  if (const char* env = getenv("E2D_ADJOINT_GATHER_ROWS")) {
+    const int r = atoi(env);
+    if (r >= 1) return r;
   }
-  const int r = static_cast<int>(rows / 170.0 * 3.0 / waves_per_simd + 0.5);
-  return r < 8 ? 8 : (r > 32 ? 32 : r);
+  return 8;
 }
 
 template <class P>
@@ -509,19 +642,24 @@ hipError_t adjoint_step(const e2d_params& p, const DevBuffers& b, int buf, doubl
   hipError_t e = launch_boundaries(p, b, buf, st);
   if (e != hipSuccess) return e;
   const Grid& g = a.g;
-  hipLaunchKernelGGL(k_adj_prim<P>, dim3((g.isize + 63) / 64, (g.jsize + 3) / 4), dim3(256), 0, st, a);
   if (adjoint_passes_separate()) {
+    hipLaunchKernelGGL(k_adj_prim<P>, dim3((g.isize + 63) / 64, (g.jsize + 3) / 4), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_adj_faces<P>, dim3((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_adj_cells<P>, dim3((g.isize - 2 + 63) / 64, (g.jsize - 2 + 3) / 4), dim3(256), 0, st,
                        a);
+    hipLaunchKernelGGL(k_adj_gather<P>, dim3((g.isize + 63) / 64, (g.jsize + 3) / 4), dim3(256), 0, st, a);
   } else {
-    a.seg_rows = march_seg_rows<P>(g.jsize - 2);
+    a.seg_rows = march_seg_rows(g.jsize - 2);
     a.nsegs = (g.jsize - 2 + a.seg_rows - 1) / a.seg_rows;
     a.seg_groups = (a.nsegs + kAdjWaves - 1) / kAdjWaves;
     const int nstrips = (g.isize - 2 + kAdjStripOut - 1) / kAdjStripOut;
     hipLaunchKernelGGL(k_adj_march<P>, dim3(nstrips * a.seg_groups), dim3(64 * kAdjWaves), 0, st, a);
+    a.gseg_rows = gather_seg_rows();
+    a.gnsegs = (g.jsize + a.gseg_rows - 1) / a.gseg_rows;
+    a.gseg_groups = (a.gnsegs + kAdjWaves - 1) / kAdjWaves;
+    const int gstrips = (g.isize + kGatherStripOut - 1) / kGatherStripOut;
+    hipLaunchKernelGGL(k_adj_gather_march<P>, dim3(gstrips * a.gseg_groups), dim3(64 * kAdjWaves), 0, st, a);
   }
-  hipLaunchKernelGGL(k_adj_gather<P>, dim3((g.isize + 63) / 64, (g.jsize + 3) / 4), dim3(256), 0, st, a);
   const int frame = 4 * g.nx + 4 * (g.ny > 4 ? g.ny - 4 : 0);
   hipLaunchKernelGGL(k_adj_fold_frame<P>, dim3((frame + 255) / 256), dim3(256), 0, st, a);
   return hipGetLastError();

@@ -203,6 +203,8 @@ def adjoint_metrics(e2d):
     # a dense seeded lam: every face carries a nonzero flux bar (a zero bar
     # skips that face's Riemann adjoint)
     run.field_upload(a, np.random.default_rng(7).normal(size=(2052, 2052, 4)))
+    run.adjoint_step(buf, dt, a, b)  # untimed: first launches load the kernels
+    run.synchronize()
     run.enable_timers(True)
     run.get_timers()
     for _ in range(20):
@@ -279,6 +281,9 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--adjoint-steps", type=int, default=0,
+                    help="only time this many adjoint steps (dense seeded lam, state after 100 steps) "
+                         "against as many forward steps at the workload size and exit")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (10 times) and exit; "
                          "its FETCH_SIZE/WRITE_SIZE counters calibrate the HBM byte counters")
@@ -307,6 +312,30 @@ def main(argv=None):
         return 1
     prec = e2d.PRECISIONS[args.precision]
     nx = args.nx
+
+    if args.adjoint_steps:
+        run = make_run(e2d, nx, 0, prec)
+        run.run_steps(100)
+        run.synchronize()
+        buf, dt = run.current(), run.get_dt()
+        a, b = run.field_alloc(), run.field_alloc()
+        run.field_upload(a, np.random.default_rng(7).normal(size=(nx + 4, nx + 4, 4)))
+        run.adjoint_step(buf, dt, a, b)  # untimed: first launches load the kernels
+        run.synchronize()
+        run.enable_timers(True)
+        run.get_timers()
+        for _ in range(args.adjoint_steps):
+            run.adjoint_step(buf, dt, a, b)
+        adj = run.get_timers()["godunov"]
+        for _ in range(args.adjoint_steps):
+            run.godunov_unsplit(run.nStep, dt)
+        fwd = run.get_timers()["godunov"]
+        run.close()
+        a_ms, f_ms = adj[0] / adj[1], fwd[0] / fwd[1]
+        print(json.dumps({"metric": f"adjoint step / forward step at {nx}^2 {args.precision}", "value": round(a_ms / f_ms, 3),
+                          "adjoint_step_ms": round(a_ms, 4), "forward_step_ms": round(f_ms, 4),
+                          "steps": args.adjoint_steps, "higher_is_better": False}))
+        return 0
 
     _log(f"[bench] {nx}x{nx} {args.precision} impl {args.impl}: {args.warmup} warmup + {args.steps} steps")
     run = make_run(e2d, nx, args.impl, prec)

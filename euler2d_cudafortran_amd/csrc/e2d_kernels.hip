@@ -38,6 +38,7 @@ constexpr int kStripOut = 60;  // output columns per wavefront
 #define E2D_WAVES_PER_BLOCK 4
 #endif
 constexpr int kWavesPerBlock = E2D_WAVES_PER_BLOCK;  // independent wavefronts per workgroup
+
 // optional register cap for the fused kernel (minimum resident waves per SIMD)
 #if defined(E2D_FUSED_MIN_WAVES) && E2D_FUSED_MIN_WAVES > 0
 #define E2D_FUSED_ATTR __attribute__((amdgpu_waves_per_eu(E2D_FUSED_MIN_WAVES)))

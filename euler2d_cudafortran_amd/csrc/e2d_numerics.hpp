@@ -250,39 +250,50 @@ E2D_HDI void godunov_flux(const Hydro<T>& h, const T g[4], T f[4]) {
 
 // Approximate two-shock Riemann solver (riemann_2d, m_utils.f90:349-492):
 // Newton iterations on p*, star state, sampling at x/t = 0, transverse
-// velocity upwinded by the contact.  Writes the Godunov state and its flux.
+// velocity upwinded by the contact.  Written as three pieces: floors and
+// first guess; one Newton iteration; star state, sampling and flux.
+template <class T>
+struct TwoShock {
+  T rl, ul, pl, rr, ur, pr, cl, cr, p_star, conv;
+};
+
 template <class T, class M>
-E2D_HDI void riemann_two_shock(const Hydro<T>& h, const T ql[4], const T qr[4],
-                                                  T g[4], T f[4]) {
-  const T rl = fmax2(ql[0], h.smallr);
-  const T ul = ql[2];
-  const T pl = fmax2(ql[1], rl * h.smallp);
-  const T rr = fmax2(qr[0], h.smallr);
-  const T ur = qr[2];
-  const T pr = fmax2(qr[1], rr * h.smallp);
-
+E2D_HDI void two_shock_start(const Hydro<T>& h, const T ql[4], const T qr[4], TwoShock<T>& s) {
+  s.rl = fmax2(ql[0], h.smallr);
+  s.ul = ql[2];
+  s.pl = fmax2(ql[1], s.rl * h.smallp);
+  s.rr = fmax2(qr[0], h.smallr);
+  s.ur = qr[2];
+  s.pr = fmax2(qr[1], s.rr * h.smallp);
   // Lagrangian sound speeds squared and first guess
-  const T cl = h.gamma0 * pl * rl;
-  const T cr = h.gamma0 * pr * rr;
-  T wl = M::sqrt(cl);
-  T wr = M::sqrt(cr);
-  T p_star = fmax2(M::div((wr * pl + wl * pr) + wl * wr * (ul - ur), wl + wr), T(0));
+  s.cl = h.gamma0 * s.pl * s.rl;
+  s.cr = h.gamma0 * s.pr * s.rr;
+  const T wl = M::sqrt(s.cl);
+  const T wr = M::sqrt(s.cr);
+  s.p_star = fmax2(M::div((wr * s.pl + wl * s.pr) + wl * wr * (s.ul - s.ur), wl + wr), T(0));
+  s.conv = T(1);
+}
 
-  T conv = T(1);
-  for (int it = 0; it < h.niter && conv > h.conv_tol; ++it) {
-    const T wwl = M::sqrt(cl * (T(1) + M::div(h.gamma6 * (p_star - pl), pl)));
-    const T wwr = M::sqrt(cr * (T(1) + M::div(h.gamma6 * (p_star - pr), pr)));
-    const T zl = M::div(T(2) * wwl * wwl * wwl, wwl * wwl + cl);
-    const T zr = M::div(T(2) * wwr * wwr * wwr, wwr * wwr + cr);
-    const T usl = ul - M::div(p_star - pl, wwl);
-    const T usr = ur + M::div(p_star - pr, wwr);
-    const T delp = fmax2(M::div(zr * zl, zr + zl) * (usl - usr), -p_star);
-    p_star = p_star + delp;
-    conv = fabs_(M::div(delp, p_star + h.smallpp));
-  }
+// one Newton iteration: the next p* and its convergence measure
+template <class T, class M>
+E2D_HDI void two_shock_newton(const Hydro<T>& h, const TwoShock<T>& s, T& p_next, T& conv_next) {
+  const T wwl = M::sqrt(s.cl * (T(1) + M::div(h.gamma6 * (s.p_star - s.pl), s.pl)));
+  const T wwr = M::sqrt(s.cr * (T(1) + M::div(h.gamma6 * (s.p_star - s.pr), s.pr)));
+  const T zl = M::div(T(2) * wwl * wwl * wwl, wwl * wwl + s.cl);
+  const T zr = M::div(T(2) * wwr * wwr * wwr, wwr * wwr + s.cr);
+  const T usl = s.ul - M::div(s.p_star - s.pl, wwl);
+  const T usr = s.ur + M::div(s.p_star - s.pr, wwr);
+  const T delp = fmax2(M::div(zr * zl, zr + zl) * (usl - usr), -s.p_star);
+  p_next = s.p_star + delp;
+  conv_next = fabs_(M::div(delp, p_next + h.smallpp));
+}
 
-  wl = M::sqrt(cl * (T(1) + M::div(h.gamma6 * (p_star - pl), pl)));
-  wr = M::sqrt(cr * (T(1) + M::div(h.gamma6 * (p_star - pr), pr)));
+template <class T, class M>
+E2D_HDI void two_shock_finish(const Hydro<T>& h, const T ql[4], const T qr[4], const TwoShock<T>& s, T g[4],
+                              T f[4]) {
+  const T rl = s.rl, ul = s.ul, pl = s.pl, rr = s.rr, ur = s.ur, pr = s.pr, p_star = s.p_star;
+  const T wl = M::sqrt(s.cl * (T(1) + M::div(h.gamma6 * (p_star - pl), pl)));
+  const T wr = M::sqrt(s.cr * (T(1) + M::div(h.gamma6 * (p_star - pr), pr)));
   const T u_star = T(0.5) * (ul + M::div(pl - p_star, wl) + ur - M::div(pr - p_star, wr));
 
   // upwind side of the contact (sgnm = +-1)
@@ -320,6 +331,15 @@ E2D_HDI void riemann_two_shock(const Hydro<T>& h, const T ql[4], const T qr[4],
   g[2] = gu;
   g[3] = right_going ? ql[3] : qr[3];
   godunov_flux(h, g, f);
+}
+
+template <class T, class M>
+E2D_HDI void riemann_two_shock(const Hydro<T>& h, const T ql[4], const T qr[4],
+                                                  T g[4], T f[4]) {
+  TwoShock<T> s;
+  two_shock_start<T, M>(h, ql, qr, s);
+  for (int it = 0; it < h.niter && s.conv > h.conv_tol; ++it) two_shock_newton<T, M>(h, s, s.p_star, s.conv);
+  two_shock_finish<T, M>(h, ql, qr, s, g, f);
 }
 
 // HLLC solver (riemann_2d_hllc, m_utils.f90:534-644).  The reference never

@@ -348,12 +348,19 @@ def main(argv=None):
     state_bytes = 8 if prec == e2d.PREC_FP64 else 4
     alg_bytes = cells * 4 * state_bytes * 2  # read + write of 4 conserved variables
     achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
-    traffic = None
+    traffic, valu = None, None
     if TRAFFIC_FILE.exists():
         tr = json.loads(TRAFFIC_FILE.read_text())
         key = f"{nx}_{args.precision}_impl{args.impl}"
         if key in tr:
             traffic = tr[key]["bytes_per_launch"]
+            if "valu_issue_us" in tr[key]:
+                # VALU issue time per launch from the counter run, against this run's kernel time
+                valu = {"bound": "fp64 valu issue", "issue_us": tr[key]["valu_issue_us"],
+                        "kernel_us": round(kernel_ms * 1e3, 1),
+                        "frac": round(tr[key]["valu_issue_us"] / (kernel_ms * 1e3), 3),
+                        "lane_instructions_per_cell": tr[key]["valu_lane_instructions_per_cell"],
+                        "source": tr[key]["valu_source"]}
     _log(f"[bench] {value:.1f} Mcell/s, {ms_per_step:.4f} ms/step, kernel {kernel_ms:.4f} ms, "
          f"{achieved:.1f} GB/s algorithmic")
 
@@ -406,8 +413,7 @@ def main(argv=None):
                    "dt_after_run": dt_final},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "limiter": ("FP64 VALU issue, not HBM (profiles/k_step_fused_fp64_2048_valu.txt)"
-                                 if (args.precision, args.impl) == ("fp64", 0) else None),
+                     "limiter": valu,
                      "kernel": {0: "k_step_fused", 1: "materialised step (5 kernels)",
                                 2: "flux-store step (3 kernels)"}[args.impl],
                      "kernel_avg_ms": round(kernel_ms, 5),

@@ -248,9 +248,10 @@ static int choose_seg_rows(const void* kernel, int ny) {
   return r < 8 ? 8 : (r > 32 ? 32 : r);
 }
 
+// per host thread, so that contexts driven from different threads do not race
 template <class P, int SOLVER>
 static int seg_rows_for(int ny) {
-  static int cached_ny = -1, cached = 16;
+  thread_local int cached_ny = -1, cached = 16;
   if (ny != cached_ny) {
     cached = choose_seg_rows(reinterpret_cast<const void*>(k_step_fused<P, SOLVER>), ny);
     cached_ny = ny;

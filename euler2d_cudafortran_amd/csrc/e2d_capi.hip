@@ -96,8 +96,13 @@ struct e2d_ctx {
 
 namespace {
 
-int check_ctx(const e2d_ctx* c) {
+// Every call on a context runs on the device that was current at e2d_create
+// (its stream and buffers live there): make it current on this thread.
+int check_ctx(e2d_ctx* c) {
   if (!c) return fail(E2D_ERR_ARG, "context is NULL");
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess || cur != c->device)
+    if (hipSetDevice(c->device) != hipSuccess) return fail(E2D_ERR_HIP, "hipSetDevice(%d) failed", c->device);
   return E2D_OK;
 }
 int check_buf(int buf) {
@@ -139,7 +144,11 @@ template <class F>
 hipError_t timed(e2d_ctx* c, int cat, F&& fn) {
   if (!c->timers_on) return fn();
   hipEvent_t a = take_event(c), b = take_event(c);
-  if (!a || !b) return hipErrorOutOfMemory;
+  if (!a || !b) {
+    if (a) c->event_pool.push_back(a);
+    if (b) c->event_pool.push_back(b);
+    return hipErrorOutOfMemory;
+  }
   hipError_t e = hipEventRecord(a, c->stream);
   if (e == hipSuccess) e = fn();
   if (e == hipSuccess) e = hipEventRecord(b, c->stream);
@@ -219,12 +228,14 @@ int e2d_create(const e2d_params* p, e2d_ctx** out) {
   int rc = validate(p);
   if (rc) return rc;
   if (!e2d_device_available()) return fail(E2D_ERR_HIP, "no HIP device available");
+  int device = 0;
+  HIP_TRY(hipGetDevice(&device));
   e2d_ctx* c = new (std::nothrow) e2d_ctx();
   if (!c) return fail(E2D_ERR_NOMEM, "host allocation failed");
   c->p = *p;
   c->isize = p->nx + 4;
   c->jsize = p->ny + 4;
-  HIP_TRY(hipGetDevice(&c->device));
+  c->device = device;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return fail(E2D_ERR_HIP, "hipStreamCreate failed");
@@ -267,6 +278,7 @@ int e2d_create(const e2d_params* p, e2d_ctx** out) {
 
 int e2d_destroy(e2d_ctx* c) {
   if (!c) return E2D_OK;
+  (void)check_ctx(c);
   (void)hipStreamSynchronize(c->stream);
   free_all(c);
   delete c;

@@ -259,10 +259,14 @@ class HydroRun:
         _check(self.lib.e2d_create(ctypes.byref(self._cp), ctypes.byref(h)))
         self.ctx = h
         self.nStep = 0
+        self._fields: set[int] = set()  # device fields from field_alloc, freed by close()
 
     # -- lifetime ---------------------------------------------------------
     def close(self):
         if getattr(self, "ctx", None):
+            for d in getattr(self, "_fields", ()):
+                self.lib.e2d_field_free(self.ctx, d)
+            self._fields = set()
             self.lib.e2d_destroy(self.ctx)
             self.ctx = None
 
@@ -363,15 +367,18 @@ class HydroRun:
         _check(self.lib.e2d_device_buffer(self.ctx, buf, ctypes.byref(p), ctypes.byref(pitch)))
         return p.value, pitch.value
 
-    # -- tangent / adjoint (FP64, two-shock solver) ------------------------
+    # -- tangent (FP64) / adjoint (all precisions), two-shock solver --------
     def field_alloc(self) -> int:
-        """Zero FP64 vector field in the device layout; returns its device address."""
+        """Zero FP64 vector field in the device layout; returns its device address
+        (freed by field_free or when the run is closed)."""
         p = ctypes.c_void_p()
         _check(self.lib.e2d_field_alloc(self.ctx, ctypes.byref(p)))
+        self._fields.add(p.value)
         return p.value
 
     def field_free(self, d: int):
         _check(self.lib.e2d_field_free(self.ctx, d))
+        self._fields.discard(d)
 
     def field_upload(self, d: int, h: np.ndarray):
         a = np.ascontiguousarray(np.asfortranarray(h, dtype=np.float64).reshape(-1, order="F"))
@@ -409,7 +416,9 @@ class HydroRun:
 
     # -- main loop of the reference (serial_ref/main.f90:42-76) ---------------
     def run_reference_loop(self, nstepmax: int, tend: float = 10.0) -> list[float]:
-        """Host-driven loop: compute_dt each step (synchronous) then godunov_unsplit."""
+        """Host-driven loop: compute_dt each step (synchronous) then godunov_unsplit.
+        t is never advanced, as in the reference's loop (serial_ref/main.f90:53-76),
+        so nstepmax alone ends it."""
         dts = []
         t = 0.0
         dt = self.compute_dt(self.nStep % 2)

@@ -110,6 +110,9 @@ int e2d_device_available(void);
 /* Message of the last failure on this thread ("" if none). */
 const char *e2d_last_error(void);
 
+/* A context lives on the device that is current when it is created; every
+ * call on it makes that device current on the calling thread. */
+
 /* Replaces initHydroParameters_gpu + the allocations of initHydroRun
  * (cuf/m_HydroParameters_gpu.cuf:67-103, cuf/m_HydroRun_gpu.cuf:83-120):
  * validates p, allocates the two state buffers (u = buffer 0, u2 = buffer 1)

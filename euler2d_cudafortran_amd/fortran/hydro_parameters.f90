@@ -5,7 +5,9 @@
 !! implementationVersion (0 fused, 1 materialised, 2 flux-store) plus this implementation's
 !! options: precision ('fp64' | 'fp32' | 'mixed'), problem ('implode' | 'diag'
 !! | 'bumps'), riemann_solver (0 two-shock, 2 HLLC), seed, device_loop
-!! (keep dt on the device between outputs) and vtk_format ('ascii' |
+!! (keep dt on the device between outputs), async_output (with device_loop:
+!! launch the steps up to the next output before writing the current one, so
+!! the file is written while the GPU computes) and vtk_format ('ascii' |
 !! 'binary').  &OTHER is optional; a file written for serial_ref runs as is.
 module HydroParameters
 
@@ -39,6 +41,7 @@ module HydroParameters
   integer :: riemann_solver = 0
   integer(8) :: seed = 0
   logical :: device_loop = .false.
+  logical :: async_output = .false.
   character(len=16) :: vtk_format = 'ascii'
 
   type(e2d_params_t) :: params
@@ -52,7 +55,8 @@ contains
     namelist/run/tEnd, nStepmax, nOutput
     namelist/mesh/nx, ny, boundary_type_xmin, boundary_type_xmax, boundary_type_ymin, boundary_type_ymax
     namelist/hydro/gamma0, cfl, smallr, smallc, niter_riemann, iorder, slope_type
-    namelist/other/implementationVersion, precision, problem, riemann_solver, seed, device_loop, vtk_format
+    namelist/other/implementationVersion, precision, problem, riemann_solver, seed, device_loop, async_output, &
+         & vtk_format
 
     narg = command_argument_count()
     if (narg /= 1) then

@@ -133,3 +133,22 @@ def test_binary_vtk_matches_ascii(gpu_lib, exe, tmp_path, kind, dtype):
         if kind == "fp32":  # the ASCII file holds the shortest decimal of each float
             a, b = a.astype(np.float32), b.astype(np.float32)
         assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+def test_async_output_writes_the_same_files(gpu_lib, exe, tmp_path):
+    """async_output (with device_loop) launches the steps up to the next output
+    before writing the current file: the files are byte-identical to the
+    serial order's, and the Perf line is still printed."""
+    files = {}
+    for mode in (".false.", ".true."):
+        wd = tmp_path / mode.strip(".")
+        wd.mkdir()
+        write_nml(wd / "t.nml", 64, 48, nstepmax=23, noutput=5,
+                  extra=f"&OTHER\nproblem='diag'\ndevice_loop=.true.\nasync_output={mode}\n/\n")
+        r = _run(["t.nml"], wd)
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert "Perf" in r.stdout
+        files[mode] = {p.name: p.read_bytes() for p in sorted(wd.glob("*.vti"))}
+    assert list(files[".true."]) == [f"euler2d_{s:08d}.vti" for s in (0, 5, 10, 15, 20, 23)]
+    assert files[".true."] == files[".false."]

@@ -341,6 +341,17 @@ def main(argv=None):
     run = make_run(e2d, nx, args.impl, prec)
     wall, kernel_ms = time_steps(run, args.steps, args.warmup)
     dt_final = run.get_dt()
+    # copy ceiling: device-to-device copies of one state buffer through the
+    # C-ABI, host-timed (each call synchronises, so this is a lower estimate)
+    dst = run.field_alloc()
+    run.copy_from_buffer(0, dst)
+    reps = 50
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run.copy_from_buffer(0, dst)
+    copy_s = (time.perf_counter() - t0) / reps
+    buf_bytes = run.device_buffer(0)[1] * (nx + 4) * 4 * (8 if prec == e2d.PREC_FP64 else 4)
+    copy_gbs = 2 * buf_bytes / copy_s / 1e9
     run.close()
     cells = nx * nx
     value = cells * args.steps / wall / 1e6
@@ -417,6 +428,7 @@ def main(argv=None):
                      "kernel": {0: "k_step_fused", 1: "materialised step (5 kernels)",
                                 2: "flux-store step (3 kernels)"}[args.impl],
                      "kernel_avg_ms": round(kernel_ms, 5),
+                     "copy_ceiling_gbs": round(copy_gbs, 1),
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "variants": variants,

@@ -232,6 +232,12 @@ __global__ __launch_bounds__(256) void k_adj_cells(const AdjArgs<P> a) {
 #else
 #define E2D_ADJ_MARCH_ATTR
 #endif
+#ifndef E2D_BAND_MARCH
+#define E2D_BAND_MARCH 1
+#endif
+#ifndef E2D_BAND_GATHER
+#define E2D_BAND_GATHER 1
+#endif
 constexpr int kAdjWaves = 4;      // wavefronts per workgroup
 constexpr int kAdjStripOut = 60;  // output columns per wavefront
 
@@ -243,8 +249,9 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
   const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int strip = lb / a.seg_groups;
-  const int seg = (lb % a.seg_groups) * kAdjWaves + wave;
+  int strip, group;
+  strip_and_group<E2D_BAND_MARCH>(lb, a.seg_groups, strip, group);
+  const int seg = group * kAdjWaves + wave;
   if (seg >= a.nsegs) return;  // whole wavefront leaves: no barrier in this kernel
 
   const Grid& g = a.g;
@@ -265,16 +272,17 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
   T qa[4], qb[4], qc[4];
   prim_at(a, cl, row(ra - 2), qa);
   prim_at(a, cl, row(ra - 1), qb);
-  // row r-1, carried: q, slopes, trace sources and the bars of its xmin, xmax, ymin face states
-  T pq[4], pdx[4], pdy[4], ps[4], pbxm[4], pbxp[4], pbym[4];
+  // row r-1, carried: slopes, trace sources and the bars of its xmin, xmax, ymin face states
+  // (its primitives are qa)
+  T pdx[4], pdy[4], ps[4], pbxm[4], pbxp[4], pbym[4];
 
   for (int r = ra - 1; r <= rb; ++r) {
     prim_at(a, cl, row(r + 1), qc);
     T qxm[4], qxp[4], dqx[4], dqy[4], s[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      qxm[v] = from_left(qb[v]);
-      qxp[v] = from_right(qb[v]);
+      qxm[v] = from_left<true>(qb[v]);
+      qxp[v] = from_right<true>(qb[v]);
     }
     cell_slopes(a.h, qb, qxp, qxm, qc, qa, dqx, dqy);
     trace_sources<T, M>(a.h, qb, dqx, dqy, s);
@@ -298,7 +306,7 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
         }
         if (any) {
           T fl[4], fr[4];
-          face_state(a.h, pq, pdy, ps, dtdy, true, fl);
+          face_state(a.h, qa, pdy, ps, dtdy, true, fl);
           face_state(a.h, qb, dqy, s, dtdy, false, fr);
           riemann_y_adj<M>(a.h, fl, fr, fb, ylb, bym);
         }
@@ -312,11 +320,11 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
           any |= (pbxm[v] != T(0)) | (pbxp[v] != T(0)) | (pbym[v] != T(0)) | (ylb[v] != T(0));
         if (any) {
           T sb[4] = {T(0), T(0), T(0), T(0)};
-          face_state_adj(a.h, pq, pdx, ps, dtdx, false, pbxm, qbar, dxb, sb);
-          face_state_adj(a.h, pq, pdx, ps, dtdx, true, pbxp, qbar, dxb, sb);
-          face_state_adj(a.h, pq, pdy, ps, dtdy, false, pbym, qbar, dyb, sb);
-          face_state_adj(a.h, pq, pdy, ps, dtdy, true, ylb, qbar, dyb, sb);
-          trace_sources_adj<M>(a.h, pq, pdx, pdy, sb, qbar, dxb, dyb);
+          face_state_adj(a.h, qa, pdx, ps, dtdx, false, pbxm, qbar, dxb, sb);
+          face_state_adj(a.h, qa, pdx, ps, dtdx, true, pbxp, qbar, dxb, sb);
+          face_state_adj(a.h, qa, pdy, ps, dtdy, false, pbym, qbar, dyb, sb);
+          face_state_adj(a.h, qa, pdy, ps, dtdy, true, ylb, qbar, dyb, sb);
+          trace_sources_adj<M>(a.h, qa, pdx, pdy, sb, qbar, dxb, dyb);
         }
         store4(a.qb, g, ci, r - 1, qbar);
         store4(a.dqxb, g, ci, r - 1, dxb);
@@ -328,7 +336,7 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
         face_state(a.h, qb, dqx, s, dtdx, true, xmax_own);
         face_state(a.h, qb, dqx, s, dtdx, false, xr);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) xl[v] = from_left(xmax_own[v]);
+        for (int v = 0; v < 4; ++v) xl[v] = from_left<true>(xmax_own[v]);
         if (xface && r >= 2 && r <= g.ny + 1) {
           T fb[4];
           bool any = false;
@@ -343,12 +351,11 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
           if (any) riemann_two_shock_adj<M>(a.h, xl, xr, fb, xlb, bxm);
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) bxp[v] = from_right(xlb[v]);
+        for (int v = 0; v < 4; ++v) bxp[v] = from_right<true>(xlb[v]);
       }
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      pq[v] = qb[v];
       pdx[v] = dqx[v];
       pdy[v] = dqy[v];
       ps[v] = s[v];
@@ -377,8 +384,9 @@ __global__ __launch_bounds__(64 * kAdjWaves) void k_adj_gather_march(const AdjAr
   const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int strip = lb / a.gseg_groups;
-  const int seg = (lb % a.gseg_groups) * kAdjWaves + wave;
+  int strip, group;
+  strip_and_group<E2D_BAND_GATHER>(lb, a.gseg_groups, strip, group);
+  const int seg = group * kAdjWaves + wave;
   if (seg >= a.gnsegs) return;  // whole wavefront leaves: no barrier in this kernel
 
   const Grid& g = a.g;
@@ -431,15 +439,15 @@ __global__ __launch_bounds__(64 * kAdjWaves) void k_adj_gather_march(const AdjAr
       for (int v = 0; v < 4; ++v) {
         xm[v] = x0[v] = xp[v] = T(0);
         xv[v] = limited && cell_has && dxb[v] != T(0);
-        const T ql = from_left(q0[v]), qr = from_right(q0[v]);
+        const T ql = from_left<true>(q0[v]), qr = from_right<true>(q0[v]);
         if (xv[v]) limited_slope_adj(st, ql, q0[v], qr, dxb[v], xm[v], x0[v], xp[v]);
       }
       if (cell_has) load4(a.qb, g, ci, r, qbx);
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const T lp = from_left(xp[v]), rm = from_right(xm[v]);
-        const bool lv = __shfl_up(static_cast<int>(xv[v]), 1, 64) != 0;
-        const bool rv = __shfl_down(static_cast<int>(xv[v]), 1, 64) != 0;
+        const T lp = from_left<true>(xp[v]), rm = from_right<true>(xm[v]);
+        const bool lv = from_left<true>(static_cast<int>(xv[v])) != 0;
+        const bool rv = from_right<true>(static_cast<int>(xv[v])) != 0;
         if (limited) {
           if (lv) qbx[v] += lp;  // left neighbour: this cell is its plus side
           if (xv[v]) qbx[v] += x0[v];

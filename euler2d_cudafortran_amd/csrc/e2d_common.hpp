@@ -141,13 +141,41 @@ __device__ __forceinline__ int64_t at(const Grid& g, int i, int j, int v) {
 
 // ---------------------------------------------------------------------------
 // cross-lane helpers (wave64)
-template <class T>
-__device__ __forceinline__ T from_left(T x) {  // value of lane-1
-  return __shfl_up(x, 1, 64);
+//
+// Neighbour exchange: from_left(x) is the value of lane l-1, from_right(x)
+// that of lane l+1, and the edge lane that has no neighbour keeps its own
+// value.  Two ways to move it: ds_bpermute (__shfl_up / __shfl_down, issued to
+// the LDS pipe, with an LDS round trip of latency) or DPP wavefront shifts
+// (wave_shr:1 / wave_shl:1 with bound_ctrl off: one v_mov_b32_dpp per 32 bits
+// on the VALU, no round trip).  The forward step is bound by VALU issue and
+// keeps its VALU slots for arithmetic (kDpp = false; DPP measured 5% slower in
+// FP32 at 2048^2); the adjoint kernels wait on latency more than on issue and
+// use DPP (kDpp = true; 1% faster in FP64, 6% in FP32).  Callers run them with
+// all 64 lanes active.
+template <bool kDpp, bool kUp>
+__device__ __forceinline__ int lane_shift(int x) {
+  if (kDpp) return __builtin_amdgcn_update_dpp(x, x, kUp ? 0x138 : 0x130, 0xf, 0xf, false);
+  return kUp ? __shfl_up(x, 1, 64) : __shfl_down(x, 1, 64);
 }
-template <class T>
+template <bool kDpp, bool kUp>
+__device__ __forceinline__ float lane_shift(float x) {
+  return __builtin_bit_cast(float, lane_shift<kDpp, kUp>(__builtin_bit_cast(int, x)));
+}
+template <bool kDpp, bool kUp>
+__device__ __forceinline__ double lane_shift(double x) {
+  if (!kDpp) return kUp ? __shfl_up(x, 1, 64) : __shfl_down(x, 1, 64);
+  const long long b = __double_as_longlong(x);
+  const int lo = lane_shift<kDpp, kUp>(static_cast<int>(b));
+  const int hi = lane_shift<kDpp, kUp>(static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+template <bool kDpp = false, class T>
+__device__ __forceinline__ T from_left(T x) {  // value of lane-1
+  return lane_shift<kDpp, true>(x);
+}
+template <bool kDpp = false, class T>
 __device__ __forceinline__ T from_right(T x) {  // value of lane+1
-  return __shfl_down(x, 1, 64);
+  return lane_shift<kDpp, false>(x);
 }
 template <class T>
 __device__ __forceinline__ T wave_max(T x) {
@@ -175,15 +203,29 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nb) {
   return (xcd < r) ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
 }
 
+// Strip and segment group of logical block lb in a grid of nstrips x
+// seg_groups blocks.  Band-major: neighbouring strips of one band of rows are
+// consecutive blocks, so the cache lines two strips share (their column halos)
+// and the rows two bands share meet in one L2 at about the same time; this
+// pays where a kernel is bound by memory traffic.  Strip-major: the segments
+// of one strip are consecutive.
+template <bool kBandMajor>
+__device__ __forceinline__ void strip_and_group(int lb, int seg_groups, int& strip, int& group) {
+  if (kBandMajor) {
+    const int nstrips = static_cast<int>(gridDim.x) / seg_groups;
+    group = lb / nstrips;
+    strip = lb % nstrips;
+  } else {
+    strip = lb / seg_groups;
+    group = lb % seg_groups;
+  }
+}
+
 
 // cross-lane shuffles of a dual number move both parts
-template <>
-__device__ __forceinline__ Dual from_left<Dual>(Dual x) {
-  return Dual(__shfl_up(x.v, 1, 64), __shfl_up(x.d, 1, 64));
-}
-template <>
-__device__ __forceinline__ Dual from_right<Dual>(Dual x) {
-  return Dual(__shfl_down(x.v, 1, 64), __shfl_down(x.d, 1, 64));
+template <bool kDpp, bool kUp>
+__device__ __forceinline__ Dual lane_shift(Dual x) {
+  return Dual(lane_shift<kDpp, kUp>(x.v), lane_shift<kDpp, kUp>(x.d));
 }
 
 }  // namespace e2d

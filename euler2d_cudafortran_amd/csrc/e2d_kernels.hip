@@ -89,6 +89,9 @@ __device__ __forceinline__ void prim_of(const Hydro<typename P::T>& h, const typ
   cons_to_prim<T, typename P::M>(h, u, q);
 }
 
+#ifndef E2D_BAND_FWD
+#define E2D_BAND_FWD 0
+#endif
 template <class P, int SOLVER>
 __global__ __launch_bounds__(64 * kWavesPerBlock) E2D_FUSED_ATTR void k_step_fused(const StepArgs<P> a) {
   using T = typename P::T;
@@ -99,8 +102,9 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) E2D_FUSED_ATTR void k_step_fus
   const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int strip = lb / a.seg_groups;
-  const int seg = (lb % a.seg_groups) * kWavesPerBlock + wave;
+  int strip, group;
+  strip_and_group<E2D_BAND_FWD>(lb, a.seg_groups, strip, group);
+  const int seg = group * kWavesPerBlock + wave;
   if (seg >= a.nsegs) return;  // whole wavefront leaves: no barrier in this kernel
 
   const Grid& g = a.g;

@@ -48,6 +48,7 @@ struct AdjArgs {
   const double* lam;
   double* out;
   T *q, *xminb, *xmaxb, *yminb, *ymaxb, *qb, *dqxb, *dqyb, *ub;
+  T* seam;  // k_adj_march_gather: partial sums of the rows where segments meet (16 planes)
   double dt;
   int nsegs, seg_rows, seg_groups;     // k_adj_march decomposition
   int gnsegs, gseg_rows, gseg_groups;  // k_adj_gather_march decomposition
@@ -491,6 +492,294 @@ __global__ __launch_bounds__(64 * kAdjWaves) void k_adj_gather_march(const AdjAr
   }
 }
 
+// Passes 2-4 in one marching kernel (the default): k_adj_march with the
+// gather folded in.  A wavefront owns a strip of 64 columns and marches up a
+// segment of rows [ra, rb).  Completions (face-state and trace-source
+// adjoints, as in k_adj_march) are valid on lanes 2..61; the gather of a cell
+// needs the completions of its x-neighbours, so gathered columns are lanes
+// 3..60 (58 per strip).  When row k is complete, its x- and y-limiter
+// adjoints are formed at once (q of rows k-1, k, k+1 are in registers); the
+// bar of q of a cell is
+//   ((own bar + x-limiter parts) + plus part from row k-1) + own y part) + minus part from row k+1
+// in k_adj_gather's order, so row k-1 is final when row k is complete.  Rows
+// ra and rb-1 miss a part that the segment below / above forms: their
+// partial sums go to the seam buffer and k_adj_seam finishes them (and rows 0
+// and jsize-1).  A part that k_adj_gather skips is carried as -0.0, the
+// identity of IEEE addition, so the sums are bit-identical to the unfused
+// passes.
+constexpr int kFusedStripOut = 58;
+// Seam buffer: 4 slots of 4 values per completed row and column, indexed by
+// the row (so that the store addresses vary with the marching loop):
+//   row ra:   0 own + x-limiter parts; 1 own y part; 2 minus part (for row ra-1)
+//   row ra+1: 0 minus part (for row ra)
+//   row rb-1: 1 sum up to the own y part (when rb-1 > ra); 3 plus part (for row rb)
+__device__ __forceinline__ int64_t seam_at(const Grid& g, int row, int slot, int v, int ci) {
+  return (static_cast<int64_t>(row * 4 + slot) * 4 + v) * g.pitch + ci;
+}
+
+// bar of u of cell (ci, row) from its final bar of q: lam_in + bar for
+// interior cells, the bar alone for ghost cells (folded by k_adj_fold_frame)
+template <class P>
+__device__ __forceinline__ void finish_cell(const AdjArgs<P>& a, int ci, int row, const typename P::T qb[4]) {
+  using T = typename P::T;
+  const Grid& g = a.g;
+  T u[4], ub[4] = {T(0), T(0), T(0), T(0)};
+  load4(a.u, g, ci, row, u);
+  cons_to_prim_adj<typename P::M>(a.h, u, qb, ub);
+  if (interior(g, ci, row)) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) a.out[at(g, ci, row, v)] = a.lam[at(g, ci, row, v)] + static_cast<double>(ub[v]);
+  } else {
+    store4(a.ub, g, ci, row, ub);
+  }
+}
+
+template <class P>
+__global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march_gather(const AdjArgs<P> a) {
+  using T = typename P::T;
+  using U = typename P::U;
+  using M = typename P::M;
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  int strip, group;
+  strip_and_group<E2D_BAND_MARCH>(lb, a.seg_groups, strip, group);
+  const int seg = group * kAdjWaves + wave;
+  if (seg >= a.nsegs) return;  // whole wavefront leaves: no barrier in this kernel
+
+  const Grid& g = a.g;
+  const int ci = strip * kFusedStripOut + lane - 3;  // column of this lane
+  const int cl = ci < 0 ? 0 : (ci > g.isize - 1 ? g.isize - 1 : ci);
+  const bool cell_col = ci >= 1 && ci <= g.isize - 2;          // the column has completions
+  const bool done_col = lane >= 2 && lane < 62 && cell_col;    // completed on this lane
+  const bool xface = lane >= 2 && lane <= 62 && ci >= 2 && ci <= g.nx + 2;  // face ci-1/2
+  const bool yface = lane >= 2 && lane < 62 && ci >= 2 && ci <= g.nx + 1;
+  const bool gat_col = lane >= 3 && lane <= 60 && ci >= 0 && ci < g.isize;  // gathered on this lane
+  const int ra = 1 + seg * a.seg_rows;
+  const int rb = min(ra + a.seg_rows, g.jsize - 1);  // completed rows [ra, rb)
+  const bool limited = slopes_limited(a.h);
+  const T st = static_cast<T>(a.h.slope_type);
+  const T skip = -T(0);
+
+  U dtdxU, dtdyU;
+  step_ratios(a, dtdxU, dtdyU);
+  const T dtdx = static_cast<T>(dtdxU), dtdy = static_cast<T>(dtdyU);
+  auto row = [&](int r) { return r < 0 ? 0 : (r > g.jsize - 1 ? g.jsize - 1 : r); };
+
+  T qz[4], qa[4], qb[4], qc[4];  // primitives of rows r-2, r-1, r, r+1
+  prim_at(a, cl, row(ra - 2), qa);
+  prim_at(a, cl, row(ra - 1), qb);
+  // row r-1, carried: slopes, trace sources and the bars of its xmin, xmax, ymin face states
+  T pdx[4], pdy[4], ps[4], pbxm[4], pbxp[4], pbym[4];
+  // gather, carried: partial sum of the last completed row and its plus part
+  T acc[4], ypf[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) qz[v] = acc[v] = ypf[v] = T(0);
+
+  for (int r = ra - 1; r <= rb; ++r) {
+    prim_at(a, cl, row(r + 1), qc);
+    T qxm[4], qxp[4], dqx[4], dqy[4], s[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      qxm[v] = from_left<true>(qb[v]);
+      qxp[v] = from_right<true>(qb[v]);
+    }
+    cell_slopes(a.h, qb, qxp, qxm, qc, qa, dqx, dqy);
+    trace_sources<T, M>(a.h, qb, dqx, dqy, s);
+
+    T bym[4], bxm[4], bxp[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) bym[v] = bxm[v] = bxp[v] = T(0);
+    if (r >= ra) {
+      // y-face between rows r-1 and r
+      T ylb[4] = {T(0), T(0), T(0), T(0)};
+      if (yface && r >= 2 && r <= g.ny + 2) {
+        T fb[4];
+        bool any = false;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          double x = 0.0;
+          if (interior(g, ci, r)) x += a.lam[at(g, ci, r, v)];
+          if (interior(g, ci, r - 1)) x -= a.lam[at(g, ci, r - 1, v)];
+          fb[v] = static_cast<T>(x * static_cast<double>(dtdyU));
+          any |= (x != 0.0);
+        }
+        if (any) {
+          T fl[4], fr[4];
+          face_state(a.h, qa, pdy, ps, dtdy, true, fl);
+          face_state(a.h, qb, dqy, s, dtdy, false, fr);
+          riemann_y_adj<M>(a.h, fl, fr, fb, ylb, bym);
+        }
+      }
+      if (r - 1 >= ra) {
+        // cell row k = r-1 is complete: bars of its q, x-slopes and y-slopes
+        const int k = r - 1;
+        T qbar[4] = {T(0), T(0), T(0), T(0)}, dxb[4] = {T(0), T(0), T(0), T(0)}, dyb[4] = {T(0), T(0), T(0), T(0)};
+        if (done_col) {
+          bool any = false;
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            any |= (pbxm[v] != T(0)) | (pbxp[v] != T(0)) | (pbym[v] != T(0)) | (ylb[v] != T(0));
+          if (any) {
+            T sb[4] = {T(0), T(0), T(0), T(0)};
+            face_state_adj(a.h, qa, pdx, ps, dtdx, false, pbxm, qbar, dxb, sb);
+            face_state_adj(a.h, qa, pdx, ps, dtdx, true, pbxp, qbar, dxb, sb);
+            face_state_adj(a.h, qa, pdy, ps, dtdy, false, pbym, qbar, dyb, sb);
+            face_state_adj(a.h, qa, pdy, ps, dtdy, true, ylb, qbar, dyb, sb);
+            trace_sources_adj<M>(a.h, qa, pdx, pdy, sb, qbar, dxb, dyb);
+          }
+        }
+        // limiter adjoints of row k (q of rows k-1, k, k+1 = qz, qa, qb); the
+        // own bar and the x parts of the cell and its x-neighbours, in k_adj_gather's order
+        T qbx[4], y0f[4], ymf[4], ypk[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const bool xv = limited && done_col && dxb[v] != T(0);
+          const T ql = from_left<true>(qa[v]), qr = from_right<true>(qa[v]);  // all lanes active here
+          T xm = T(0), x0 = T(0), xp = T(0);
+          if (xv) limited_slope_adj(st, ql, qa[v], qr, dxb[v], xm, x0, xp);
+          const T lp = from_left<true>(xp), rm = from_right<true>(xm);
+          const bool lv = from_left<true>(static_cast<int>(xv)) != 0;
+          const bool rv = from_right<true>(static_cast<int>(xv)) != 0;
+          qbx[v] = qbar[v];
+          if (limited) {
+            if (lv) qbx[v] += lp;  // left neighbour: this cell is its plus side
+            if (xv) qbx[v] += x0;
+            if (rv) qbx[v] += rm;  // right neighbour: this cell is its minus side
+          }
+          const bool yv = limited && done_col && dyb[v] != T(0);
+          T ym = T(0), y0 = T(0), yp = T(0);
+          if (yv) limited_slope_adj(st, qz[v], qa[v], qb[v], dyb[v], ym, y0, yp);
+          ymf[v] = yv ? ym : skip;
+          y0f[v] = yv ? y0 : skip;
+          ypk[v] = yv ? yp : skip;
+        }
+        // seam row index through readfirstlane (k is wave-uniform): an opaque
+        // scalar, so the compiler forms each store address from it when the
+        // store runs instead of carrying 16 address registers through the loop
+        const int ks = __builtin_amdgcn_readfirstlane(k);
+        if (gat_col) {
+          if (k >= ra + 2) {
+            // row k-1 is final
+            T qfin[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) qfin[v] = acc[v] + ymf[v];
+            finish_cell(a, ci, ks - 1, qfin);
+          }
+          if (k == ra) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              a.seam[seam_at(g, ks, 0, v, ci)] = qbx[v];
+              a.seam[seam_at(g, ks, 1, v, ci)] = y0f[v];
+              a.seam[seam_at(g, ks, 2, v, ci)] = ymf[v];
+            }
+          }
+          if (k == ra + 1) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) a.seam[seam_at(g, ks, 0, v, ci)] = ymf[v];
+          }
+        }
+        // partial sum of row k: ((own + x parts) + plus part of row k-1) + own y part
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[v] = (qbx[v] + ypf[v]) + y0f[v];
+        if (gat_col && k == rb - 1) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            if (k > ra) a.seam[seam_at(g, ks, 1, v, ci)] = acc[v];
+            a.seam[seam_at(g, ks, 3, v, ci)] = ypk[v];
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) ypf[v] = ypk[v];
+      }
+      // x-faces of row r
+      if (r < rb) {
+        T xmax_own[4], xl[4], xr[4], xlb[4] = {T(0), T(0), T(0), T(0)};
+        face_state(a.h, qb, dqx, s, dtdx, true, xmax_own);
+        face_state(a.h, qb, dqx, s, dtdx, false, xr);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) xl[v] = from_left<true>(xmax_own[v]);
+        if (xface && r >= 2 && r <= g.ny + 1) {
+          T fb[4];
+          bool any = false;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            double x = 0.0;
+            if (interior(g, ci, r)) x += a.lam[at(g, ci, r, v)];
+            if (interior(g, ci - 1, r)) x -= a.lam[at(g, ci - 1, r, v)];
+            fb[v] = static_cast<T>(x * static_cast<double>(dtdxU));
+            any |= (x != 0.0);
+          }
+          if (any) riemann_two_shock_adj<M>(a.h, xl, xr, fb, xlb, bxm);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) bxp[v] = from_right<true>(xlb[v]);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      pdx[v] = dqx[v];
+      pdy[v] = dqy[v];
+      ps[v] = s[v];
+      pbxm[v] = bxm[v];
+      pbxp[v] = bxp[v];
+      pbym[v] = bym[v];
+      qz[v] = qa[v];
+      qa[v] = qb[v];
+      qb[v] = qc[v];
+    }
+  }
+}
+
+// The rows k_adj_march_gather leaves unfinished: row 0, row jsize-1, and the
+// first and last row of every segment, from the seam buffer.  Thread row t:
+// 0 -> row 0, 1 -> row jsize-1, 2 + 2s -> first row of segment s, 3 + 2s ->
+// last row of segment s (none when the segment has one row).
+template <class P>
+__global__ __launch_bounds__(256) void k_adj_seam(const AdjArgs<P> a) {
+  using T = typename P::T;
+  const Grid& g = a.g;
+  const int ci = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int t = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (ci >= g.isize || t >= 2 + 2 * a.nsegs) return;
+  const T skip = -T(0);
+  auto V = [&](int row, int slot, int v) { return a.seam[seam_at(g, row, slot, v, ci)]; };
+  const int last = g.jsize - 2;  // last completed row
+  T qb[4];
+  int r;
+  if (t == 0) {
+    r = 0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) qb[v] = T(0) + V(1, 2, v);
+  } else if (t == 1) {
+    r = g.jsize - 1;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) qb[v] = T(0) + V(last, 3, v);
+  } else {
+    const int sg = (t - 2) >> 1;
+    const int ra = 1 + sg * a.seg_rows;
+    const int rb = min(ra + a.seg_rows, g.jsize - 1);
+    if (((t - 2) & 1) == 0) {
+      r = ra;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const T below = ra > 1 ? V(ra - 1, 3, v) : skip;
+        const T above = ra + 1 < rb ? V(ra + 1, 0, v) : (rb <= last ? V(rb, 2, v) : skip);
+        qb[v] = ((V(ra, 0, v) + below) + V(ra, 1, v)) + above;
+      }
+    } else {
+      r = rb - 1;
+      if (r == ra) return;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const T above = rb <= last ? V(rb, 2, v) : skip;
+        qb[v] = V(r, 1, v) + above;
+      }
+    }
+  }
+  finish_cell(a, ci, r, qb);
+}
+
 // all cells: bar of q from the cell's own trace and the limiters of the cell
 // and its four neighbours, then bar of u
 template <class P>
@@ -594,12 +883,29 @@ __global__ __launch_bounds__(256) void k_adj_fold_frame(const AdjArgs<P> a) {
   for (int v = 0; v < 4; ++v) a.out[at(g, i, j, v)] = r[v];
 }
 
-// E2D_ADJOINT_PASSES=separate runs passes 2 and 3 as the two gather kernels
-// k_adj_faces / k_adj_cells (face bars through HBM) instead of k_adj_march;
-// both give the same bits.
-bool adjoint_passes_separate() {
-  const char* env = getenv("E2D_ADJOINT_PASSES");
-  return env && strcmp(env, "separate") == 0;
+// Kernel sequence of an adjoint step, all giving the same bits:
+//   fused   k_adj_march_gather + k_adj_seam
+//   march   k_adj_march + k_adj_gather_march (bars of q and of the slopes
+//           through HBM)
+//   separate  k_adj_prim, k_adj_faces, k_adj_cells, k_adj_gather (face bars
+//           through HBM as well)
+// E2D_ADJOINT_PASSES=fused|march|separate chooses; by default FP32 arithmetic
+// (fp32 and mixed) runs fused and FP64 runs march: the fused FP64 kernel needs
+// more than 256 VGPRs and its spills make it slower (2048^2, dense lam: FP64
+// 841 us fused, 812 us march; FP32 487 us fused, 535 us march).  The fused
+// kernel writes lam_out while other wavefronts still read lam_in, so an
+// aliased call (lam_out == lam_in) runs march, whose gather reads and writes
+// each element of lam in the same thread.
+enum class Passes { kFused, kMarch, kSeparate };
+Passes adjoint_passes(bool fp64, bool aliased) {
+  Passes p = fp64 ? Passes::kMarch : Passes::kFused;
+  if (const char* env = getenv("E2D_ADJOINT_PASSES")) {
+    if (strcmp(env, "separate") == 0) p = Passes::kSeparate;
+    if (strcmp(env, "march") == 0) p = Passes::kMarch;
+    if (strcmp(env, "fused") == 0) p = Passes::kFused;
+  }
+  if (p == Passes::kFused && aliased) p = Passes::kMarch;
+  return p;
 }
 
 // Rows per k_adj_march segment (E2D_ADJOINT_SEG_ROWS overrides): about
@@ -646,16 +952,25 @@ hipError_t adjoint_step(const e2d_params& p, const DevBuffers& b, int buf, doubl
   a.dqxb = static_cast<T*>(s.dqxb);
   a.dqyb = static_cast<T*>(s.dqyb);
   a.ub = static_cast<T*>(s.ub);
+  a.seam = static_cast<T*>(s.seam);
   a.dt = dt;
   hipError_t e = launch_boundaries(p, b, buf, st);
   if (e != hipSuccess) return e;
   const Grid& g = a.g;
-  if (adjoint_passes_separate()) {
+  const Passes passes = adjoint_passes(sizeof(T) == sizeof(double), lam_in == lam_out);
+  if (passes == Passes::kSeparate) {
     hipLaunchKernelGGL(k_adj_prim<P>, dim3((g.isize + 63) / 64, (g.jsize + 3) / 4), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_adj_faces<P>, dim3((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_adj_cells<P>, dim3((g.isize - 2 + 63) / 64, (g.jsize - 2 + 3) / 4), dim3(256), 0, st,
                        a);
     hipLaunchKernelGGL(k_adj_gather<P>, dim3((g.isize + 63) / 64, (g.jsize + 3) / 4), dim3(256), 0, st, a);
+  } else if (passes == Passes::kFused) {
+    a.seg_rows = march_seg_rows(g.jsize - 2);
+    a.nsegs = (g.jsize - 2 + a.seg_rows - 1) / a.seg_rows;
+    a.seg_groups = (a.nsegs + kAdjWaves - 1) / kAdjWaves;
+    const int nstrips = (g.isize + kFusedStripOut - 1) / kFusedStripOut;
+    hipLaunchKernelGGL(k_adj_march_gather<P>, dim3(nstrips * a.seg_groups), dim3(64 * kAdjWaves), 0, st, a);
+    hipLaunchKernelGGL(k_adj_seam<P>, dim3((g.isize + 63) / 64, (2 + 2 * a.nsegs + 3) / 4), dim3(256), 0, st, a);
   } else {
     a.seg_rows = march_seg_rows(g.jsize - 2);
     a.nsegs = (g.jsize - 2 + a.seg_rows - 1) / a.seg_rows;

@@ -92,6 +92,7 @@ struct e2d_ctx {
   void* dual[2] = {nullptr, nullptr};
   e2d::AdjointScratch adj;
   std::vector<void*> snapshots;
+  double* lam_tmp = nullptr;  // the rollout's second adjoint field
 };
 
 namespace {
@@ -180,11 +181,12 @@ void free_adjoint(e2d_ctx* c) {
   for (auto& d : c->dual)
     if (d) (void)hipFree(d), d = nullptr;
   void** arrs[] = {&c->adj.q, &c->adj.xminb, &c->adj.xmaxb, &c->adj.yminb, &c->adj.ymaxb,
-                   &c->adj.qb, &c->adj.dqxb, &c->adj.dqyb, &c->adj.ub};
+                   &c->adj.qb, &c->adj.dqxb, &c->adj.dqyb, &c->adj.ub, &c->adj.seam};
   for (void** a : arrs)
     if (*a) (void)hipFree(*a), *a = nullptr;
   for (void* sn : c->snapshots) (void)hipFree(sn);
   c->snapshots.clear();
+  if (c->lam_tmp) (void)hipFree(c->lam_tmp), c->lam_tmp = nullptr;
 }
 
 void free_all(e2d_ctx* c) {
@@ -516,14 +518,16 @@ int ensure_adjoint_scratch(e2d_ctx* c) {
   if (c->adj.q) return E2D_OK;
   const size_t bytes = static_cast<size_t>(c->b.plane) * 4 * e2d::adjoint_scratch_elem_bytes(c->p.precision);
   void** arrs[] = {&c->adj.q, &c->adj.xminb, &c->adj.xmaxb, &c->adj.yminb, &c->adj.ymaxb,
-                   &c->adj.qb, &c->adj.dqxb, &c->adj.dqyb, &c->adj.ub};
+                   &c->adj.qb, &c->adj.dqxb, &c->adj.dqyb, &c->adj.ub, &c->adj.seam};
   for (void** a : arrs) {
-    if (hipMalloc(a, bytes) != hipSuccess) {
+    // the seam buffer holds 16 planes, the others 4
+    const size_t n = a == &c->adj.seam ? 4 * bytes : bytes;
+    if (hipMalloc(a, n) != hipSuccess) {
       free_adjoint(c);
       return fail(E2D_ERR_NOMEM, "adjoint scratch allocation failed");
     }
     // slots never written by a pass must read as zero
-    HIP_TRY(hipMemsetAsync(*a, 0, bytes, c->stream));
+    HIP_TRY(hipMemsetAsync(*a, 0, n, c->stream));
   }
   return E2D_OK;
 }
@@ -556,7 +560,11 @@ struct Rollout {
   e2d_ctx* c;
   int n0;
   const double* dts;
-  double* lam;
+  // lam ping-pongs between two fields (lam[cur] holds the current one), so
+  // that no adjoint step runs in place; both start as lam_n, and ghost cells,
+  // which no step writes, keep lam_n's values in either
+  double* lam[2];
+  int cur = 0;
   std::vector<int> free_slots;
   int64_t nfwd = 0;
   // step whose state each buffer holds (-1: none of this rollout).  A step
@@ -594,7 +602,8 @@ struct Rollout {
       for (int k = b - 1; k >= a; --k) {
         if (int rc = restore(slot, a)) return rc;
         if (int rc = advance(a, k)) return rc;
-        if (int rc = adjoint_step(c, buf_of(k), dts[k], lam, lam)) return rc;
+        if (int rc = adjoint_step(c, buf_of(k), dts[k], lam[cur], lam[1 - cur])) return rc;
+        cur = 1 - cur;
       }
       return E2D_OK;
     }
@@ -657,13 +666,17 @@ int e2d_adjoint_rollout(e2d_ctx* c, int nStep0, int nsteps, const double* dts, i
       return fail(E2D_ERR_NOMEM, "snapshot %zu allocation failed", c->snapshots.size());
     c->snapshots.push_back(sn);
   }
+  if (!c->lam_tmp && hipMalloc(reinterpret_cast<void**>(&c->lam_tmp), field_bytes(c)) != hipSuccess)
+    return fail(E2D_ERR_NOMEM, "adjoint field allocation failed");
   if (lam_0 != lam_n)
     HIP_TRY(hipMemcpyAsync(lam_0, lam_n, field_bytes(c), hipMemcpyDeviceToDevice, c->stream));
-  Rollout r{c, nStep0, dts, lam_0, {}, 0};
+  HIP_TRY(hipMemcpyAsync(c->lam_tmp, lam_n, field_bytes(c), hipMemcpyDeviceToDevice, c->stream));
+  Rollout r{c, nStep0, dts, {lam_0, c->lam_tmp}};
   r.held[r.buf_of(0)] = 0;
   for (int k = nsnap - 1; k >= 1; --k) r.free_slots.push_back(k);
   if (int rc = r.save(0, 0)) return rc;
   if (int rc = r.reverse(0, nsteps, 0)) return rc;
+  if (r.cur == 1) HIP_TRY(hipMemcpyAsync(lam_0, c->lam_tmp, field_bytes(c), hipMemcpyDeviceToDevice, c->stream));
   if (int rc = r.restore(0, 0)) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (n_forward) *n_forward = r.nfwd;

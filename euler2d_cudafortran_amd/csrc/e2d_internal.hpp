@@ -71,6 +71,7 @@ hipError_t launch_tangent_step(const e2d_params& p, const DevBuffers& b, int buf
 struct AdjointScratch {
   void *q = nullptr, *xminb = nullptr, *xmaxb = nullptr, *yminb = nullptr, *ymaxb = nullptr;
   void *qb = nullptr, *dqxb = nullptr, *dqyb = nullptr, *ub = nullptr;
+  void* seam = nullptr;  // 16 planes: partial sums where adjoint row segments meet
 };
 size_t adjoint_scratch_elem_bytes(int precision);
 hipError_t launch_adjoint_step(const e2d_params& p, const DevBuffers& b, int buf, double dt,

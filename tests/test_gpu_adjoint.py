@@ -378,17 +378,28 @@ def test_fp32_rollout_against_fp64(gpu_lib):
 
 @pytest.mark.parametrize("prec", [e2d.PREC_FP64, e2d.PREC_FP32, e2d.PREC_MIXED])
 def test_marching_adjoint_equals_separate_passes(gpu_lib, prec, monkeypatch):
-    """The marching adjoint kernel (face bars in registers) against the two
-    gather passes (face bars through HBM, E2D_ADJOINT_PASSES=separate): the
-    same bits, on odd sizes, thin grids and every boundary type."""
-    for bc, (nx, ny) in [((1, 1, 1, 1), (96, 80)), ((3, 3, 3, 3), (61, 37)), ((1, 3, 2, 1), (130, 7)),
-                         ((2, 2, 2, 2), (4, 4)), ((1, 1, 3, 3), (64, 200))]:
+    """The three kernel sequences of the adjoint step give the same bits: the
+    fused marching kernel (face bars and the gather in registers, segment seams
+    through a small buffer), the marching kernel plus a marching gather (bars
+    of q and of the slopes through HBM) and the gather passes
+    (E2D_ADJOINT_PASSES=separate, face bars through HBM as well), on odd sizes,
+    thin grids, every boundary type, and row segments of 1, 2, 3 and 5 rows
+    (the last segment of a grid may be shorter than the others); and each in
+    place (lam_out = lam_in, which runs march where fused was chosen)."""
+    cases = [((1, 1, 1, 1), (96, 80), None), ((3, 3, 3, 3), (61, 37), None), ((1, 3, 2, 1), (130, 7), None),
+             ((2, 2, 2, 2), (4, 4), None), ((1, 1, 3, 3), (64, 200), None), ((1, 1, 1, 1), (96, 80), "3"),
+             ((3, 1, 2, 3), (70, 21), "2"), ((1, 2, 3, 1), (33, 45), "1"), ((2, 3, 1, 1), (50, 60), "5")]
+    for bc, (nx, ny), seg_rows in cases:
         p = _params(nx, ny, precision=prec, boundary_type_xmin=bc[0], boundary_type_xmax=bc[1],
                     boundary_type_ymin=bc[2], boundary_type_ymax=bc[3])
         lam = np.random.default_rng(nx * ny).normal(size=(nx + 4, ny + 4, 4))
         res = {}
-        for mode in ("separate", "march"):
+        for mode in ("separate", "march", "fused"):
             monkeypatch.setenv("E2D_ADJOINT_PASSES", mode)
+            if seg_rows is None:
+                monkeypatch.delenv("E2D_ADJOINT_SEG_ROWS", raising=False)
+            else:
+                monkeypatch.setenv("E2D_ADJOINT_SEG_ROWS", seg_rows)
             with e2d.HydroRun(p) as run:
                 U = _evolved_diag(run, 25)
                 run.upload(0, U)
@@ -397,4 +408,15 @@ def test_marching_adjoint_equals_separate_passes(gpu_lib, prec, monkeypatch):
                 run.field_upload(a, lam)
                 run.adjoint_step(0, dt, a, b)
                 res[mode] = run.field_download(b)
-        assert np.array_equal(res["separate"], res["march"]), f"bc={bc} {nx}x{ny}"
+                # in place (lam_out aliasing lam_in): interior bit-identical, ghost
+                # cells keep lam_in's values
+                run.adjoint_step(0, dt, a, a)
+                res[mode + " in place"] = run.field_download(a)
+        interior = np.zeros(lam.shape, bool)
+        interior[2:-2, 2:-2] = True
+        for mode in ("march", "fused"):
+            assert np.array_equal(res["separate"], res[mode]), f"{mode} bc={bc} {nx}x{ny} seg_rows={seg_rows}"
+        for mode in ("separate", "march", "fused"):
+            got = res[mode + " in place"]
+            assert np.array_equal(got[interior], res["separate"][interior]), f"{mode} in place bc={bc} {nx}x{ny}"
+            assert np.array_equal(got[~interior], lam[~interior]), f"{mode} in place ghosts bc={bc} {nx}x{ny}"

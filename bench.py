@@ -184,8 +184,8 @@ def _rollout(e2d, nx, prec, dts, nsnap, lam_of):
 
 def adjoint_metrics(e2d):
     """Adjoint cost and accuracy.
-    * one adjoint step (dense seeded lam) against one forward step at 2048^2
-      FP64 (kernel time from hipEvents, state after 100 steps);
+    * one adjoint step (dense seeded lam) against one forward step at 2048^2,
+      FP64, FP32 and mixed (kernel time from hipEvents, state after 100 steps);
     * 512^2 x 400 steps, 20 snapshots, J = sum of rho over [0.6,0.8]^2 (FP64);
     * 2048^2 x 2000 steps, 45 snapshots, J = sum over the interior of
       (rho_T - rho_target)^2 with a seeded smooth target: FP64 and FP32
@@ -195,28 +195,29 @@ def adjoint_metrics(e2d):
     Forward times are the step launches alone (dt frozen), gradient times the
     whole checkpointed reverse sweep including recomputation."""
     out = {}
-    run = make_run(e2d, 2048, 0, e2d.PREC_FP64)
-    run.run_steps(100)
-    run.synchronize()
-    buf, dt = run.current(), run.get_dt()
-    a, b = run.field_alloc(), run.field_alloc()
-    # a dense seeded lam: every face carries a nonzero flux bar (a zero bar
-    # skips that face's Riemann adjoint)
-    run.field_upload(a, np.random.default_rng(7).normal(size=(2052, 2052, 4)))
-    run.adjoint_step(buf, dt, a, b)  # untimed: first launches load the kernels
-    run.synchronize()
-    run.enable_timers(True)
-    run.get_timers()
-    for _ in range(20):
-        run.godunov_unsplit(run.nStep, dt)
-    fwd = run.get_timers()["godunov"]
-    for _ in range(20):
-        run.adjoint_step(buf, dt, a, b)
-    adj = run.get_timers()["godunov"]
-    run.close()
-    f_ms, a_ms = fwd[0] / fwd[1], adj[0] / adj[1]
-    out["step_2048_fp64"] = {"forward_kernel_ms": round(f_ms, 4), "adjoint_kernels_ms": round(a_ms, 4),
-                             "adjoint_over_forward": round(a_ms / f_ms, 3)}
+    for name, prec in (("fp64", e2d.PREC_FP64), ("fp32", e2d.PREC_FP32), ("mixed", e2d.PREC_MIXED)):
+        run = make_run(e2d, 2048, 0, prec)
+        run.run_steps(100)
+        run.synchronize()
+        buf, dt = run.current(), run.get_dt()
+        a, b = run.field_alloc(), run.field_alloc()
+        # a dense seeded lam: every face carries a nonzero flux bar (a zero bar
+        # skips that face's Riemann adjoint)
+        run.field_upload(a, np.random.default_rng(7).normal(size=(2052, 2052, 4)))
+        run.adjoint_step(buf, dt, a, b)  # untimed: first launches load the kernels
+        run.synchronize()
+        run.enable_timers(True)
+        run.get_timers()
+        for _ in range(20):
+            run.godunov_unsplit(run.nStep, dt)
+        fwd = run.get_timers()["godunov"]
+        for _ in range(20):
+            run.adjoint_step(buf, dt, a, b)
+        adj = run.get_timers()["godunov"]
+        run.close()
+        f_ms, a_ms = fwd[0] / fwd[1], adj[0] / adj[1]
+        out[f"step_2048_{name}"] = {"forward_kernel_ms": round(f_ms, 4), "adjoint_kernels_ms": round(a_ms, 4),
+                                    "adjoint_over_forward": round(a_ms / f_ms, 3)}
 
     def record_dts(nx, nsteps):
         run = make_run(e2d, nx, 0, e2d.PREC_FP64)

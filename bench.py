@@ -91,7 +91,8 @@ def cpu_baseline(nx: int, steps: int) -> dict:
                 "sample": f"{nx}x{nx} FP64, {steps} steps of the reference serial solver "
                           f"(serial_ref/ built with flang -O3, FP64 kind), wall {float(m.group(1)):.1f} s"}
     sys.path.insert(0, str(ROOT / "tests"))
-    from oracle_lib import IC_DIAG, Oracle
+    from oracle_lib import IC_DIAG, Oracle, lib
+    cores = lib().oracle_threads(1)
     o = Oracle(nx)
     o.init(IC_DIAG)
     t0 = time.perf_counter()
@@ -100,6 +101,25 @@ def cpu_baseline(nx: int, steps: int) -> dict:
     return {"value": nx * nx * steps / wall / 1e6, "unit": "Mcell-updates/s", "cores": cores,
             "kind": "port", "sample": f"{nx}x{nx} FP64, {steps} steps of the oracle C restatement, "
                                       f"wall {wall:.1f} s"}
+
+
+def cpu_all_cores(nx: int, steps: int) -> dict:
+    """The oracle C restatement on every host core (OpenMP over rows; the same
+    bits as one core), nx^2 for `steps` steps: a second, separately reported
+    CPU figure beside the single-core reference baseline."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_lib import IC_DIAG, Oracle, lib
+    threads = lib().oracle_threads(0)
+    o = Oracle(nx)
+    o.init(IC_DIAG)
+    o.run(2)  # untimed: first touches of the arrays
+    t0 = time.perf_counter()
+    o.run(steps)
+    wall = time.perf_counter() - t0
+    lib().oracle_threads(1)
+    return {"value": nx * nx * steps / wall / 1e6, "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{nx}x{nx} FP64, {steps} steps of the oracle C restatement on {threads} threads, "
+                      f"wall {wall:.1f} s"}
 
 
 def shutil_which(name):
@@ -399,10 +419,12 @@ def main(argv=None):
         _log("[bench] adjoint cost")
         adjoint = adjoint_metrics(e2d)
 
-    cpu = None
+    cpu = cpu_all = None
     if not args.no_cpu_baseline:
         _log(f"[bench] cpu baseline {nx}x{nx}, {args.cpu_steps} steps")
         cpu = cpu_baseline(nx, args.cpu_steps)
+        _log(f"[bench] oracle on all host cores {nx}x{nx}, 20 steps")
+        cpu_all = cpu_all_cores(nx, 20)
 
     line = {
         "metric": "Mcell-updates/s at 2048^2 FP64 vs FP32 (%HBM peak); adjoint/forward cost ratio",
@@ -432,6 +454,7 @@ def main(argv=None):
                      "copy_ceiling_gbs": round(copy_gbs, 1),
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
+        "cpu_all_cores": cpu_all,
         "variants": variants,
         "adjoint": adjoint,
     }

@@ -9,6 +9,23 @@
 
 #include "cpu_ref.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Rows of the grid loops run in parallel (every cell is computed alone; the
+ * dt reduction is a max), which changes no result.  n > 0 sets the thread
+ * count; returns the count that calls will use. */
+int oracle_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
 /* variable and face indices: serial_ref/m_constants.f90:11-14, 51-54 */
 enum { ID = 1, IP = 2, IU = 3, IV = 4 };
 enum { FACE_XMIN = 0, FACE_XMAX = 1, FACE_YMIN = 2, FACE_YMAX = 3 };

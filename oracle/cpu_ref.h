@@ -49,6 +49,9 @@ enum {
 ORACLE_DECLARE(f64)
 ORACLE_DECLARE(f32)
 
+/* thread count of the grid loops (n > 0 sets it); returns the count in use */
+int oracle_threads(int n);
+
 #undef ORACLE_DECLARE
 
 #ifdef __cplusplus

@@ -369,6 +369,10 @@ double FN(oracle_compute_dt)(void *hv, int useU) {
   FN(run) *h = (FN(run) *)hv;
   const int ghostWidth = 2;
   R *a = (useU == 0) ? h->u : h->u2;
+  /* the running max is the largest of invDt and every rate, whatever the
+   * order, so rows may run in parallel */
+  R m = h->invDt;
+#pragma omp parallel for schedule(static) reduction(max : m)
   for (int j = ghostWidth + 1; j <= h->jsize - ghostWidth - 1; ++j)
     for (int i = ghostWidth + 1; i <= h->isize - ghostWidth - 1; ++i) {
       R uLoc[4] = {AT(a, i, j, ID), AT(a, i, j, IP), AT(a, i, j, IU), AT(a, i, j, IV)};
@@ -376,8 +380,9 @@ double FN(oracle_compute_dt)(void *hv, int useU) {
       FN(prim)(h, uLoc, qLoc, &c);
       R vx = c + ABS(qLoc[2]);
       R vy = c + ABS(qLoc[3]);
-      h->invDt = MAX2(h->invDt, vx / h->dx + vy / h->dy);
+      m = MAX2(m, vx / h->dx + vy / h->dy);
     }
+  h->invDt = m;
   return (double)(h->cfl / h->invDt);
 }
 
@@ -454,6 +459,7 @@ void FN(oracle_make_boundaries)(void *hv, int which) {
 
 /* convertToPrimitives: serial_ref/m_run.f90:317-358 (all cells) */
 static void FN(convert_to_primitives)(FN(run) * h, const R *ud) {
+#pragma omp parallel for schedule(static)
   for (int j = 1; j <= h->jsize; ++j)
     for (int i = 1; i <= h->isize; ++i) {
       R uLoc[4] = {AT(ud, i, j, ID), AT(ud, i, j, IP), AT(ud, i, j, IU), AT(ud, i, j, IV)};
@@ -478,6 +484,7 @@ static void FN(compute_and_store_fluxes)(FN(run) * h, R dt) {
   const int gw = 2;
   const R dtdx = dt / h->dx;
   const R dtdy = dt / h->dy;
+#pragma omp parallel for schedule(static)
   for (int j = gw + 1; j <= h->jsize - gw + 1; ++j)
     for (int i = gw + 1; i <= h->isize - gw + 1; ++i) {
       R qLoc[4], qLocN[4], qN0[4], qN1[4], qN2[4], qN3[4], dq[8], dqN[8];
@@ -522,6 +529,7 @@ static void FN(compute_and_store_fluxes)(FN(run) * h, R dt) {
 /* updateHydro: serial_ref/m_run.f90:489-518 (interior only) */
 static void FN(update_hydro)(FN(run) * h, const R *din, R *dout) {
   const int gw = 2;
+#pragma omp parallel for schedule(static)
   for (int j = gw + 1; j <= h->jsize - gw; ++j)
     for (int i = gw + 1; i <= h->isize - gw; ++i)
       for (int v = 1; v <= NBVAR; ++v) {

@@ -57,6 +57,8 @@ def lib():
             getattr(L, f"oracle_get_{s}").argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
             getattr(L, f"oracle_set_{s}").argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
             getattr(L, f"oracle_kat_{s}").argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.oracle_threads.restype = ctypes.c_int
+        L.oracle_threads.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
 

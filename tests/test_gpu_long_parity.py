@@ -1,0 +1,71 @@
+"""The benchmark configuration end to end: 2048^2 FP64, diagonal-discontinuity
+problem, reflective walls, 1000 steps, implementationVersion 0, 1 and 2 on the
+GPU (device-resident dt loop), against the oracle on every host core (its grid
+loops run rows in parallel, which changes no bit: each cell is computed alone
+and the dt reduction is a max).  The reference needs about 35 minutes of one
+core for this run.
+
+Bar: the stated tolerance rel-L1 <= 1e-12 per conserved variable, and in fact
+bit identity (a zero may differ in sign) of both state buffers, ghost cells and
+corners included, and of the running invDt and the dt of the next step.
+
+On a 16-core host the oracle takes about a minute (52 s measured) and the GPU
+1.4 s for the three versions; E2D_LONG_PARITY=0 skips it.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+import euler2d_cudafortran_amd as e2d
+from oracle_lib import IC_DIAG, U, U2, Oracle, lib, rel_l1
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(os.environ.get("E2D_LONG_PARITY") == "0",
+                                 reason="E2D_LONG_PARITY=0 skips the long run")]
+
+NX, STEPS = 2048, 1000
+
+
+def _same(a, b):
+    na, nb = np.isnan(a), np.isnan(b)
+    return bool(np.array_equal(na, nb) and np.all(a[~na] == b[~nb]))
+
+
+def test_2048_fp64_1000_steps_vs_oracle(gpu_lib):
+    gpu = {}
+    for impl in (0, 1, 2):
+        p = e2d.HydroParams(nx=NX, ny=NX, implementation_version=impl, precision=e2d.PREC_FP64)
+        with e2d.HydroRun(p) as run:
+            run.init("diag")
+            run.make_boundaries(0)
+            t0 = time.perf_counter()
+            run.run_steps(STEPS)
+            run.synchronize()
+            wall = time.perf_counter() - t0
+            gpu[impl] = (run.download(0), run.download(1), run.get_dt(), run.invdt)
+        print(f"GPU implementationVersion {impl}: {STEPS} steps in {wall:.2f} s", flush=True)
+
+    threads = lib().oracle_threads(0)
+    o = Oracle(NX)
+    o.init(IC_DIAG)
+    t0 = time.perf_counter()
+    # the reference main loop (serial_ref/main.f90:36-76), as oracle_run runs it
+    o.compute_dt(0)
+    o.make_boundaries(0)
+    for n in range(STEPS):
+        dt = o.compute_dt(n % 2)
+        o.godunov(n, dt)
+        if (n + 1) % 100 == 0:
+            print(f"oracle step {n + 1}/{STEPS}: {time.perf_counter() - t0:.0f} s on {threads} threads", flush=True)
+    ref_u, ref_u2 = o.get(U), o.get(U2)
+    next_dt = o.compute_dt(STEPS % 2)  # the dt the next step would use, as e2d_get_dt reports it
+    ref_invdt = o.invdt
+
+    for impl, (u, u2, gdt, ginvdt) in gpu.items():
+        errs = [rel_l1(u[2:-2, 2:-2, v], ref_u[2:-2, 2:-2, v]) for v in range(4)]
+        print(f"implementationVersion {impl}: rel-L1 per variable {errs}", flush=True)
+        assert max(errs) <= 1e-12
+        assert gdt == next_dt and ginvdt == ref_invdt, (impl, gdt, next_dt, ginvdt, ref_invdt)
+        assert _same(u, ref_u) and _same(u2, ref_u2), f"implementationVersion {impl} differs in some bit"

@@ -92,7 +92,9 @@ __device__ __forceinline__ void prim_of(const Hydro<typename P::T>& h, const typ
 #ifndef E2D_BAND_FWD
 #define E2D_BAND_FWD 0
 #endif
-template <class P, int SOLVER>
+// kMinmod: instantiated for slope_type == 1 (the benchmark's minmod), whose
+// limiter then skips the products by st = 1 and the first-order select
+template <class P, int SOLVER, bool kMinmod = false>
 __global__ __launch_bounds__(64 * kWavesPerBlock) E2D_FUSED_ATTR void k_step_fused(const StepArgs<P> a) {
   using T = typename P::T;
   using U = typename P::U;
@@ -165,7 +167,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) E2D_FUSED_ATTR void k_step_fus
       qxm[v] = from_left(qb[v]);
       qxp[v] = from_right(qb[v]);
     }
-    cell_slopes(h, qb, qxp, qxm, qc, qa, dqx, dqy);
+    cell_slopes<kMinmod>(h, qb, qxp, qxm, qc, qa, dqx, dqy);
     trace_sources<T, M>(h, qb, dqx, dqy, s);
 
     // y-face between rows r-1 and r
@@ -253,11 +255,11 @@ static int choose_seg_rows(const void* kernel, int ny) {
 }
 
 // per host thread, so that contexts driven from different threads do not race
-template <class P, int SOLVER>
+template <class P, int SOLVER, bool kMinmod = false>
 static int seg_rows_for(int ny) {
   thread_local int cached_ny = -1, cached = 16;
   if (ny != cached_ny) {
-    cached = choose_seg_rows(reinterpret_cast<const void*>(k_step_fused<P, SOLVER>), ny);
+    cached = choose_seg_rows(reinterpret_cast<const void*>(k_step_fused<P, SOLVER, kMinmod>), ny);
     cached_ny = ny;
   }
   return cached;
@@ -278,13 +280,16 @@ static hipError_t fused_step(const e2d_params& p, const DevBuffers& b, const Ste
   a.slot_next = c.slot_next;
   a.dt_host = c.dt_host;
   const bool hllc = p.riemann_solver == E2D_RIEMANN_HLLC;
-  a.seg_rows = hllc ? seg_rows_for<P, 2>(p.ny) : seg_rows_for<P, 0>(p.ny);
+  const bool minmod = !hllc && p.slope_type == 1;
+  a.seg_rows = hllc ? seg_rows_for<P, 2>(p.ny) : (minmod ? seg_rows_for<P, 0, true>(p.ny) : seg_rows_for<P, 0>(p.ny));
   a.nsegs = (p.ny + a.seg_rows - 1) / a.seg_rows;
   a.seg_groups = (a.nsegs + kWavesPerBlock - 1) / kWavesPerBlock;
   const int nstrips = (p.nx + kStripOut - 1) / kStripOut;
   const int nblocks = nstrips * a.seg_groups;
   if (hllc)
     hipLaunchKernelGGL((k_step_fused<P, 2>), dim3(nblocks), dim3(64 * kWavesPerBlock), 0, st, a);
+  else if (minmod)
+    hipLaunchKernelGGL((k_step_fused<P, 0, true>), dim3(nblocks), dim3(64 * kWavesPerBlock), 0, st, a);
   else
     hipLaunchKernelGGL((k_step_fused<P, 0>), dim3(nblocks), dim3(64 * kWavesPerBlock), 0, st, a);
   return hipGetLastError();

@@ -180,23 +180,39 @@ E2D_HDI T cfl_rate(const Hydro<T>& h, const T u[4]) {
 }
 
 // Limited slope of one variable along one direction (slope_unsplit_hydro_2d,
-// m_utils.f90:298-312): minmod for slope_type 1, MC for 2.
-template <class T>
-E2D_HDI T limited_slope(T st, T qm, T q0, T qp) {
-  const T dl = st * (q0 - qm);
-  const T dr = st * (qp - q0);
+// m_utils.f90:298-312): minmod for slope_type 1, MC for 2.  kUnit: st = 1,
+// where st * x is x itself (exactly, signed zeros and NaN included), so the two
+// products are skipped.
+template <bool kUnit, class T>
+E2D_HDI T limited_slope_st(T st, T qm, T q0, T qp) {
+  const T dl = kUnit ? (q0 - qm) : st * (q0 - qm);
+  const T dr = kUnit ? (qp - q0) : st * (qp - q0);
   const T dc = T(0.5) * (qp - qm);
   T lim = fmin2(fabs_(dl), fabs_(dr));
   if ((dl * dr) <= T(0)) lim = T(0);
   return signed_by(dc >= T(0), fmin2(lim, fabs_(dc)));
 }
+template <class T>
+E2D_HDI T limited_slope(T st, T qm, T q0, T qp) {
+  return limited_slope_st<false>(st, qm, q0, qp);
+}
 
 // Slopes of a cell in both directions.  slope_type 0 and values other than
-// 1/2 give zero slopes (first order), m_utils.f90:289-337.
-template <class T>
+// 1/2 give zero slopes (first order), m_utils.f90:289-337.  kMinmod: the
+// caller knows slope_type == 1, so the limiter skips the products by st = 1
+// and the first-order select.
+template <bool kMinmod = false, class T>
 E2D_HDI void cell_slopes(const Hydro<T>& h, const T q[4], const T qxp[4],
                                             const T qxm[4], const T qyp[4], const T qym[4],
                                             T dqx[4], T dqy[4]) {
+  if (kMinmod) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      dqx[v] = limited_slope_st<true>(T(1), qxm[v], q[v], qxp[v]);
+      dqy[v] = limited_slope_st<true>(T(1), qym[v], q[v], qyp[v]);
+    }
+    return;
+  }
   const bool limited = (h.slope_type == 1 || h.slope_type == 2);
   const T st = T(h.slope_type);
 #pragma unroll

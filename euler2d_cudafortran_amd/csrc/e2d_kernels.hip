@@ -488,6 +488,7 @@ struct MatArgs {
   const double* scal;
   int dt_from_device, slot_cur;
   double dt_host;
+  int nsegs, seg_rows, seg_groups;  // k_flux_store row segments
 };
 
 template <class P>
@@ -600,56 +601,84 @@ __global__ __launch_bounds__(256) void k_flux(const MatArgs<P> a) {
   for (int v = 0; v < 4; ++v) a.fy[at(a.g, i, j, v)] = static_cast<U>(f[v]) * dtdy;
 }
 
-// implementationVersion 2: fluxes straight from q, with the slopes of the
-// cell and of its i-1 and j-1 neighbours evaluated in registers and only fx, fy
-// stored (the structure of computeFluxesAndStore, cuf/m_HydroRun_gpu.cuf:747-884;
-// the arithmetic of computeAndStoreFluxes, serial_ref/m_run.f90:389-480).
-// Faces (i, j) in [2, size-2]^2.
-template <class P>
-__device__ __forceinline__ void cell_faces_from_q(const MatArgs<P>& a, int i, int j, typename P::T q[4],
-                                                  typename P::T dx[4], typename P::T dy[4], typename P::T s[4]) {
-  using T = typename P::T;
-  T xp[4], xm[4], yp[4], ym[4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    q[v] = a.q[at(a.g, i, j, v)];
-    xp[v] = a.q[at(a.g, i + 1, j, v)];
-    xm[v] = a.q[at(a.g, i - 1, j, v)];
-    yp[v] = a.q[at(a.g, i, j + 1, v)];
-    ym[v] = a.q[at(a.g, i, j - 1, v)];
-  }
-  cell_slopes(a.hT, q, xp, xm, yp, ym, dx, dy);
-  trace_sources<T, typename P::M>(a.hT, q, dx, dy, s);
-}
-
-template <class P, int SOLVER>
-__global__ __launch_bounds__(256) void k_flux_store(const MatArgs<P> a) {
+// computeFluxesAndStore (implementationVersion 2, cuf/m_HydroRun_gpu.cuf:747-884,
+// with serial_ref's arithmetic): fx(i, j) (face between i-1 and i) and fy(i, j)
+// (face between j-1 and j) for (i, j) in [2, isize-2]^2, times dt/dx and dt/dy,
+// from the primitives k_prim stored.  The reference computes slopes and traces
+// three times per cell (own, i-1, j-1); here, as in the fused step, a wavefront
+// owns a strip of 64 columns (60 stored, lanes 2..61) and marches up a segment
+// of rows, forming each cell's slopes, trace sources and face states once: the
+// left state of an x-face comes from lane-1, the lower state of a y-face from
+// the previous row, in registers.  Same bits as the per-cell formulation.
+template <class P, int SOLVER, bool kMinmod = false>
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_flux_store(const MatArgs<P> a) {
   using T = typename P::T;
   using U = typename P::U;
-  const int i = 2 + blockIdx.x * 64 + (threadIdx.x & 63);
-  const int j = 2 + blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (i > a.g.isize - 2 || j > a.g.jsize - 2) return;
+  using M = typename P::M;
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  int strip, group;
+  strip_and_group<false>(lb, a.seg_groups, strip, group);
+  const int seg = group * kWavesPerBlock + wave;
+  if (seg >= a.nsegs) return;  // whole wavefront leaves: no barrier in this kernel
+
+  const Grid& g = a.g;
+  const int ci = strip * kStripOut + lane;  // column of this lane
+  const int cl = ci < g.isize ? ci : g.isize - 1;
+  const bool out_lane = lane >= 2 && lane < 62 && ci <= g.isize - 2;
+  const int j0 = 2 + seg * a.seg_rows;
+  const int j1 = min(j0 + a.seg_rows, g.jsize - 1);  // stored rows [j0, j1)
   U dtdxU, dtdyU;
   step_dt(a, dtdxU, dtdyU);
   const T dtdx = static_cast<T>(dtdxU), dtdy = static_cast<T>(dtdyU);
-  T q[4], dx[4], dy[4], s[4];
-  T qn[4], dxn[4], dyn[4], sn[4];
-  T l[4], r[4], gd[4], f[4];
-  cell_faces_from_q(a, i, j, q, dx, dy, s);
-  // x face between i-1 and i
-  cell_faces_from_q(a, i - 1, j, qn, dxn, dyn, sn);
-  face_state(a.hT, qn, dxn, sn, dtdx, true, l);
-  face_state(a.hT, q, dx, s, dtdx, false, r);
-  riemann<T, typename P::M, SOLVER>(a.hT, l, r, gd, f);
+  const Hydro<T>& h = a.hT;
+
+  T qa[4], qb[4], qc[4], ymax_prev[4];
 #pragma unroll
-  for (int v = 0; v < 4; ++v) a.fx[at(a.g, i, j, v)] = static_cast<U>(f[v]) * dtdxU;
-  // y face between j-1 and j
-  cell_faces_from_q(a, i, j - 1, qn, dxn, dyn, sn);
-  face_state(a.hT, qn, dyn, sn, dtdy, true, l);
-  face_state(a.hT, q, dy, s, dtdy, false, r);
-  riemann_y<T, typename P::M, SOLVER>(a.hT, l, r, f);
+  for (int v = 0; v < 4; ++v) {
+    qa[v] = a.q[at(g, cl, j0 - 2, v)];
+    qb[v] = a.q[at(g, cl, j0 - 1, v)];
+  }
+  for (int r = j0 - 1; r < j1; ++r) {
 #pragma unroll
-  for (int v = 0; v < 4; ++v) a.fy[at(a.g, i, j, v)] = static_cast<U>(f[v]) * dtdyU;
+    for (int v = 0; v < 4; ++v) qc[v] = a.q[at(g, cl, r + 1, v)];
+    T qxm[4], qxp[4], dqx[4], dqy[4], s[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      qxm[v] = from_left(qb[v]);
+      qxp[v] = from_right(qb[v]);
+    }
+    cell_slopes<kMinmod>(h, qb, qxp, qxm, qc, qa, dqx, dqy);
+    trace_sources<T, M>(h, qb, dqx, dqy, s);
+    if (r >= j0) {
+      // y-face between rows r-1 and r
+      T ymin[4], f[4];
+      face_state(h, qb, dqy, s, dtdy, false, ymin);
+      riemann_y<T, M, SOLVER>(h, ymax_prev, ymin, f);
+      if (out_lane) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) a.fy[at(g, ci, r, v)] = static_cast<U>(f[v]) * dtdyU;
+      }
+      // x-face between columns ci-1 and ci
+      T xmin[4], xmax[4], xl[4], gd[4];
+      face_state(h, qb, dqx, s, dtdx, false, xmin);
+      face_state(h, qb, dqx, s, dtdx, true, xmax);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) xl[v] = from_left(xmax[v]);
+      riemann<T, M, SOLVER>(h, xl, xmin, gd, f);
+      if (out_lane) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) a.fx[at(g, ci, r, v)] = static_cast<U>(f[v]) * dtdxU;
+      }
+    }
+    face_state(h, qb, dqy, s, dtdy, true, ymax_prev);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      qa[v] = qb[v];
+      qb[v] = qc[v];
+    }
+  }
 }
 
 // updateHydro over the interior (serial_ref/m_run.f90:504-513)
@@ -696,10 +725,19 @@ static hipError_t materialised_step(const e2d_params& p, const DevBuffers& b, co
   const dim3 fgrid((S_ - 3 + 63) / 64, (J_ - 3 + 3) / 4);
   hipLaunchKernelGGL(k_prim<P>, dim3((S_ + 63) / 64, (J_ + 3) / 4), dim3(256), 0, st, a);
   if (p.implementation_version == E2D_IMPL_FLUX_STORE) {
-    if (p.riemann_solver == E2D_RIEMANN_HLLC)
-      hipLaunchKernelGGL((k_flux_store<P, 2>), fgrid, dim3(256), 0, st, a);
+    // stored rows and columns [2, size-2]; segment length as for the fused step
+    const bool hllc = p.riemann_solver == E2D_RIEMANN_HLLC;
+    const bool minmod = !hllc && p.slope_type == 1;
+    a.seg_rows = hllc ? seg_rows_for<P, 2>(p.ny) : (minmod ? seg_rows_for<P, 0, true>(p.ny) : seg_rows_for<P, 0>(p.ny));
+    a.nsegs = (J_ - 3 + a.seg_rows - 1) / a.seg_rows;
+    a.seg_groups = (a.nsegs + kWavesPerBlock - 1) / kWavesPerBlock;
+    const dim3 mgrid(((S_ - 3 + kStripOut - 1) / kStripOut) * a.seg_groups);
+    if (hllc)
+      hipLaunchKernelGGL((k_flux_store<P, 2>), mgrid, dim3(64 * kWavesPerBlock), 0, st, a);
+    else if (minmod)
+      hipLaunchKernelGGL((k_flux_store<P, 0, true>), mgrid, dim3(64 * kWavesPerBlock), 0, st, a);
     else
-      hipLaunchKernelGGL((k_flux_store<P, 0>), fgrid, dim3(256), 0, st, a);
+      hipLaunchKernelGGL((k_flux_store<P, 0>), mgrid, dim3(64 * kWavesPerBlock), 0, st, a);
   } else {
     hipLaunchKernelGGL(k_slopes<P>, dim3((S_ - 2 + 63) / 64, (J_ - 2 + 3) / 4), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_trace<P>, dim3((S_ - 2 + 63) / 64, (J_ - 2 + 3) / 4), dim3(256), 0, st, a);

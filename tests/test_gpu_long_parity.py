@@ -1,6 +1,7 @@
-"""The benchmark configuration end to end: 2048^2 FP64, diagonal-discontinuity
-problem, reflective walls, 1000 steps, implementationVersion 0, 1 and 2 on the
-GPU (device-resident dt loop), against the oracle on every host core (its grid
+"""The benchmark configuration end to end: 2048^2, diagonal-discontinuity
+problem, reflective walls, 1000 steps, FP64 and FP32 (the reference's default
+kind), implementationVersion 0, 1 and 2 on the GPU (device-resident dt loop),
+against the oracle of the same kind on every host core (its grid
 loops run rows in parallel, which changes no bit: each cell is computed alone
 and the dt reduction is a max).  The reference needs about 35 minutes of one
 core for this run.
@@ -9,8 +10,8 @@ Bar: the stated tolerance rel-L1 <= 1e-12 per conserved variable, and in fact
 bit identity (a zero may differ in sign) of both state buffers, ghost cells and
 corners included, and of the running invDt and the dt of the next step.
 
-On a 16-core host the oracle takes about a minute (52 s measured) and the GPU
-1.4 s for the three versions; E2D_LONG_PARITY=0 skips it.
+On a 16-core host the FP64 oracle takes about a minute (52 s measured) and the
+GPU 1.4 s for the three versions; E2D_LONG_PARITY=0 skips it.
 """
 import os
 import time
@@ -33,10 +34,11 @@ def _same(a, b):
     return bool(np.array_equal(na, nb) and np.all(a[~na] == b[~nb]))
 
 
-def test_2048_fp64_1000_steps_vs_oracle(gpu_lib):
+@pytest.mark.parametrize("prec, kind", [(e2d.PREC_FP64, "f64"), (e2d.PREC_FP32, "f32")])
+def test_2048_1000_steps_vs_oracle(gpu_lib, prec, kind):
     gpu = {}
     for impl in (0, 1, 2):
-        p = e2d.HydroParams(nx=NX, ny=NX, implementation_version=impl, precision=e2d.PREC_FP64)
+        p = e2d.HydroParams(nx=NX, ny=NX, implementation_version=impl, precision=prec)
         with e2d.HydroRun(p) as run:
             run.init("diag")
             run.make_boundaries(0)
@@ -48,7 +50,7 @@ def test_2048_fp64_1000_steps_vs_oracle(gpu_lib):
         print(f"GPU implementationVersion {impl}: {STEPS} steps in {wall:.2f} s", flush=True)
 
     threads = lib().oracle_threads(0)
-    o = Oracle(NX)
+    o = Oracle(NX, precision=kind)
     o.init(IC_DIAG)
     t0 = time.perf_counter()
     # the reference main loop (serial_ref/main.f90:36-76), as oracle_run runs it

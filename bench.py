@@ -322,8 +322,9 @@ def main(argv=None):
         return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU; a step does not shard across GPUs here "
-              "(the halo-exchange design is described in DESIGN.md). Use --gpus 1.", file=sys.stderr)
+        print("bench.py: the step runs on one GPU; it is not sharded across GPUs "
+              "(a 2-D block decomposition with a 2-cell halo exchange would do that). Use --gpus 1.",
+              file=sys.stderr)
         return 2
 
     import euler2d_cudafortran_amd as e2d

@@ -177,16 +177,20 @@ def precision_error(e2d, nx, steps):
     return out
 
 
-def _rollout(e2d, nx, prec, dts, nsnap, lam_of):
+def _rollout(e2d, nx, prec, dts, nsnap, lam_of, recorded=False):
     """Forward with the frozen dt sequence, then the checkpointed adjoint of
     lam_of(final state); returns (forward s, gradient s, recomputed steps,
-    gradient array, final state)."""
+    gradient array, final state).  recorded: the forward is
+    e2d_rollout_forward, which keeps the reverse sweep's first checkpoints."""
     run = make_run(e2d, nx, 0, prec)
     u0 = run.download(0)
     run.synchronize()
     t0 = time.perf_counter()
-    for n, d in enumerate(dts):
-        run.godunov_unsplit(n, d)
+    if recorded:
+        run.rollout_forward(0, dts, nsnap)
+    else:
+        for n, d in enumerate(dts):
+            run.godunov_unsplit(n, d)
     run.synchronize()
     t_fwd = time.perf_counter() - t0
     uT = run.download(len(dts) % 2)
@@ -212,7 +216,8 @@ def adjoint_metrics(e2d):
       (forward and adjoint in float, lam accumulated in double), same frozen
       dt sequence (recorded from the FP64 run); the FP32 gradient's relative
       L2 error against the FP64 one; and the FP64 gradient again with 200
-      snapshots (27 GB of HBM), which recomputes fewer forward steps.
+      snapshots (27 GB of HBM), which recomputes fewer forward steps, and
+      after a recorded forward (e2d_rollout_forward), with 45 and 200.
     Forward times are the step launches alone (dt frozen), gradient times the
     whole checkpointed reverse sweep including recomputation."""
     out = {}
@@ -291,6 +296,14 @@ def adjoint_metrics(e2d):
                                 "forward_steps_recomputed": nf,
                                 "same_gradient_as_45_snapshots": bool(np.array_equal(g[2:nx + 2, 2:nx + 2, :],
                                                                                      grads["fp64"]))}
+    # the forward that keeps the reverse sweep's first checkpoints (what the
+    # torch harness runs), 45 and 200 snapshots
+    for ns in (45, 200):
+        t_f, t_a, nf, g, _ = _rollout(e2d, nx, e2d.PREC_FP64, dts, ns, misfit_lam, recorded=True)
+        c5[f"fp64_recorded_forward_{ns}_snapshots"] = {
+            "forward_s": round(t_f, 4), "gradient_s": round(t_a, 4), "gradient_over_forward": round(t_a / t_f, 3),
+            "forward_steps_recomputed": nf,
+            "same_gradient_as_45_snapshots": bool(np.array_equal(g[2:nx + 2, 2:nx + 2, :], grads["fp64"]))}
     d = grads["fp32"] - grads["fp64"]
     c5["fp32_gradient_rel_l2_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(grads["fp64"]))
     c5["fp32_gradient_rel_l2_vs_fp64_per_variable"] = [

@@ -3,8 +3,9 @@
 `GodunovRollout.apply(u0, run, dts, nsnap)` maps the initial state u0 (FP64
 CUDA tensor of shape (4, jsize, pitch): the library's device layout, see
 `state_tensor`) to the state after len(dts) steps with the frozen dt
-sequence dts.  Its backward is e2d_adjoint_rollout (binomial checkpointing
-with nsnap snapshots).  Torch only allocates and moves memory here; every
+sequence dts.  Its forward is e2d_rollout_forward, which keeps the first
+checkpoints of the reverse sweep, and its backward e2d_adjoint_rollout
+(binomial checkpointing with nsnap snapshots), which starts from them.  Torch only allocates and moves memory here; every
 step and every adjoint step runs in libeuler2d_hip.  Ghost and corner entries
 of the gradient are zero (ghosts follow the interior through the boundary
 conditions; corners are treated as constants).
@@ -54,8 +55,8 @@ class GodunovRollout(torch.autograd.Function):
         u0 = u0.contiguous()
         torch.cuda.synchronize()
         run.copy_to_buffer(0, u0.data_ptr())
-        for n, dt in enumerate(dts):
-            run.godunov_unsplit(n, float(dt))
+        # the steps, keeping the snapshots the backward's reverse sweep starts from
+        run.rollout_forward(0, [float(d) for d in dts], int(nsnap))
         out = torch.empty_like(u0)
         run.copy_from_buffer(len(dts) % 2, out.data_ptr())
         ctx.run, ctx.dts, ctx.nsnap = run, [float(d) for d in dts], int(nsnap)

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/euler2d_hip.h"
@@ -93,6 +94,14 @@ struct e2d_ctx {
   e2d::AdjointScratch adj;
   std::vector<void*> snapshots;
   double* lam_tmp = nullptr;  // the rollout's second adjoint field
+  double* d_flag = nullptr;   // one device word for comparisons
+  // states of the last e2d_rollout_forward, kept in the snapshots for the
+  // next e2d_adjoint_rollout with the same arguments
+  struct Recording {
+    bool valid = false;
+    int n0 = 0, nsteps = 0, nsnap = 0;
+    std::vector<double> dts;
+  } rec;
 };
 
 namespace {
@@ -186,7 +195,9 @@ void free_adjoint(e2d_ctx* c) {
     if (*a) (void)hipFree(*a), *a = nullptr;
   for (void* sn : c->snapshots) (void)hipFree(sn);
   c->snapshots.clear();
+  c->rec.valid = false;
   if (c->lam_tmp) (void)hipFree(c->lam_tmp), c->lam_tmp = nullptr;
+  if (c->d_flag) (void)hipFree(c->d_flag), c->d_flag = nullptr;
 }
 
 void free_all(e2d_ctx* c) {
@@ -556,6 +567,31 @@ double beta(int s, int t) {
   return r;
 }
 
+// (step, slot) of the snapshots that Rollout::reverse stores on its first
+// descent from (0, nsteps) with nsnap slots: the same split and slot order
+std::vector<std::pair<int, int>> first_descent(int nsteps, int nsnap) {
+  std::vector<std::pair<int, int>> out;
+  std::vector<int> free_slots;
+  for (int k = nsnap - 1; k >= 1; --k) free_slots.push_back(k);
+  int a = 0;
+  const int b = nsteps;
+  for (;;) {
+    const int l = b - a;
+    const int s = static_cast<int>(free_slots.size());
+    if (l == 1 || s == 0) break;
+    int t = 1;
+    while (beta(s, t) < l) ++t;
+    int m = static_cast<int>(l - beta(s - 1, t));
+    if (m < 1) m = 1;
+    if (m > l - 1) m = l - 1;
+    const int mine = free_slots.back();
+    free_slots.pop_back();
+    out.emplace_back(a + m, mine);
+    a += m;
+  }
+  return out;
+}
+
 struct Rollout {
   e2d_ctx* c;
   int n0;
@@ -572,6 +608,10 @@ struct Rollout {
   // rebuilds from the interior, are rewritten), so after advancing to k the
   // other buffer still holds k-1 and a restore of a held state is skipped.
   int held[2] = {-1, -1};
+
+  // snapshots already stored by e2d_rollout_forward: (step, slot) of the
+  // first descent, consumed in order
+  std::vector<std::pair<int, int>> pre;
 
   int buf_of(int k) const { return (n0 + k) % 2; }
   int save(int slot, int k) {
@@ -612,11 +652,15 @@ struct Rollout {
     int m = static_cast<int>(l - beta(s - 1, t));
     if (m < 1) m = 1;
     if (m > l - 1) m = l - 1;
-    if (int rc = restore(slot, a)) return rc;
-    if (int rc = advance(a, a + m)) return rc;
     const int mine = free_slots.back();
     free_slots.pop_back();
-    if (int rc = save(mine, a + m)) return rc;
+    if (!pre.empty() && pre.front() == std::make_pair(a + m, mine)) {
+      pre.erase(pre.begin());  // recorded by the forward
+    } else {
+      if (int rc = restore(slot, a)) return rc;
+      if (int rc = advance(a, a + m)) return rc;
+      if (int rc = save(mine, a + m)) return rc;
+    }
     if (int rc = reverse(a + m, b, mine)) return rc;
     free_slots.push_back(mine);
     return reverse(a, a + m, slot);
@@ -671,15 +715,68 @@ int e2d_adjoint_rollout(e2d_ctx* c, int nStep0, int nsteps, const double* dts, i
   if (lam_0 != lam_n)
     HIP_TRY(hipMemcpyAsync(lam_0, lam_n, field_bytes(c), hipMemcpyDeviceToDevice, c->stream));
   HIP_TRY(hipMemcpyAsync(c->lam_tmp, lam_n, field_bytes(c), hipMemcpyDeviceToDevice, c->stream));
+  // a recording of this rollout by e2d_rollout_forward, from the same starting state?
+  bool recorded = c->rec.valid && c->rec.n0 == nStep0 && c->rec.nsteps == nsteps && c->rec.nsnap == nsnap &&
+                  std::equal(c->rec.dts.begin(), c->rec.dts.end(), dts);
+  c->rec.valid = false;
+  if (recorded) {
+    if (!c->d_flag && hipMalloc(reinterpret_cast<void**>(&c->d_flag), sizeof(double)) != hipSuccess)
+      return fail(E2D_ERR_NOMEM, "flag allocation failed");
+    HIP_TRY(e2d::launch_words_differ(c->snapshots[0], c->b.u[nStep0 % 2], state_bytes(c), c->d_flag, c->stream));
+    double differ = 1.0;
+    HIP_TRY(hipMemcpyAsync(&differ, c->d_flag, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    recorded = differ == 0.0;
+  }
   Rollout r{c, nStep0, dts, {lam_0, c->lam_tmp}};
-  r.held[r.buf_of(0)] = 0;
   for (int k = nsnap - 1; k >= 1; --k) r.free_slots.push_back(k);
-  if (int rc = r.save(0, 0)) return rc;
+  if (recorded) {
+    // snapshot 0 and the first descent's snapshots hold their states; the
+    // buffers are not trusted (the caller may have stepped since)
+    r.pre = first_descent(nsteps, nsnap);
+  } else {
+    r.held[r.buf_of(0)] = 0;
+    if (int rc = r.save(0, 0)) return rc;
+  }
   if (int rc = r.reverse(0, nsteps, 0)) return rc;
   if (r.cur == 1) HIP_TRY(hipMemcpyAsync(lam_0, c->lam_tmp, field_bytes(c), hipMemcpyDeviceToDevice, c->stream));
   if (int rc = r.restore(0, 0)) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (n_forward) *n_forward = r.nfwd;
+  return E2D_OK;
+}
+
+int e2d_rollout_forward(e2d_ctx* c, int nStep0, int nsteps, const double* dts, int nsnap) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = check_linearisable(c, false)) return rc;
+  if (nStep0 < 0 || nsteps < 1) return fail(E2D_ERR_ARG, "need nStep0 >= 0 and nsteps >= 1");
+  if (nsnap < 1) return fail(E2D_ERR_ARG, "need at least one snapshot");
+  if (!dts) return fail(E2D_ERR_ARG, "NULL argument");
+  for (int k = 0; k < nsteps; ++k)
+    if (!(dts[k] > 0.0)) return fail(E2D_ERR_ARG, "dts[%d] must be > 0", k);
+  c->rec.valid = false;
+  while (static_cast<int>(c->snapshots.size()) < nsnap) {
+    void* sn = nullptr;
+    if (hipMalloc(&sn, state_bytes(c)) != hipSuccess)
+      return fail(E2D_ERR_NOMEM, "snapshot %zu allocation failed", c->snapshots.size());
+    c->snapshots.push_back(sn);
+  }
+  const std::vector<std::pair<int, int>> plan = first_descent(nsteps, nsnap);
+  Rollout r{c, nStep0, dts, {nullptr, nullptr}};
+  if (int rc = r.save(0, 0)) return rc;
+  size_t next = 0;
+  for (int k = 0; k < nsteps; ++k) {
+    if (int rc = forward_step(c, nStep0 + k, dts[k])) return rc;
+    if (next < plan.size() && plan[next].first == k + 1) {
+      if (int rc = r.save(plan[next].second, k + 1)) return rc;
+      ++next;
+    }
+  }
+  c->rec.valid = true;
+  c->rec.n0 = nStep0;
+  c->rec.nsteps = nsteps;
+  c->rec.nsnap = nsnap;
+  c->rec.dts.assign(dts, dts + nsteps);
   return E2D_OK;
 }
 

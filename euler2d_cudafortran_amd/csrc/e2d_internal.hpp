@@ -47,6 +47,8 @@ hipError_t launch_boundaries(const e2d_params& p, const DevBuffers& b, int buf, 
 // atomic max of the CFL rate over the compute_dt window of buffer buf into scal[slot]
 hipError_t launch_max_invdt(const e2d_params& p, const DevBuffers& b, int buf, int slot, hipStream_t s);
 hipError_t launch_fill(double* dst, int n, double v, hipStream_t s);
+// *flag (device) = 1 when the two device arrays of `bytes` bytes differ in any bit, else 0
+hipError_t launch_words_differ(const void* a, const void* b, size_t bytes, double* flag, hipStream_t s);
 hipError_t launch_init_problem(const e2d_params& p, const DevBuffers& b, int problem,
                                const double* bumps /* 16 x 5 host values */, hipStream_t s);
 hipError_t launch_zero_state(const e2d_params& p, const DevBuffers& b, int buf, hipStream_t s);

@@ -472,6 +472,24 @@ hipError_t launch_fill(double* dst, int n, double v, hipStream_t s) {
   return hipGetLastError();
 }
 
+// *flag = 1 when two device arrays of n 64-bit words differ in any bit
+__global__ void k_words_differ(const unsigned long long* a, const unsigned long long* b, int64_t n, double* flag) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += stride)
+    if (a[t] != b[t]) *flag = 1.0;
+}
+
+hipError_t launch_words_differ(const void* a, const void* b, size_t bytes, double* flag, hipStream_t s) {
+  hipError_t e = launch_fill(flag, 1, 0.0, s);
+  if (e != hipSuccess) return e;
+  const int64_t n = static_cast<int64_t>(bytes / sizeof(unsigned long long));
+  const int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(k_words_differ, dim3(static_cast<unsigned>(blocks < 4096 ? (blocks > 0 ? blocks : 1) : 4096)),
+                     dim3(256), 0, s, static_cast<const unsigned long long*>(a),
+                     static_cast<const unsigned long long*>(b), n, flag);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // materialised steps (implementationVersion 1: q, slopes, face states and
 // fluxes in HBM; implementationVersion 2: q and fluxes in HBM)

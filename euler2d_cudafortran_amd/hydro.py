@@ -88,6 +88,8 @@ _SIGS = {
     "e2d_adjoint_rollout": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                            ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.POINTER(ctypes.c_int64)]),
+    "e2d_rollout_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                           ctypes.c_int]),
     "e2d_field_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     "e2d_field_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "e2d_field_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
@@ -403,6 +405,13 @@ class HydroRun:
         _check(self.lib.e2d_adjoint_rollout(self.ctx, n_step0, d.size, d.ctypes.data, nsnap, d_lam_n, d_lam_0,
                                             ctypes.byref(nf)))
         return nf.value
+
+    def rollout_forward(self, n_step0: int, dts, nsnap: int):
+        """len(dts) frozen-dt steps from the state in buffer n_step0 % 2, keeping in
+        the nsnap snapshots the states that the next adjoint_rollout with the same
+        arguments (and the same starting state back in its buffer) starts from."""
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        _check(self.lib.e2d_rollout_forward(self.ctx, n_step0, d.size, d.ctypes.data, nsnap))
 
     def copy_to_buffer(self, buf: int, d_src: int):
         _check(self.lib.e2d_copy_to_buffer(self.ctx, buf, d_src))

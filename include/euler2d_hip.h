@@ -229,6 +229,18 @@ int e2d_adjoint_step(e2d_ctx *ctx, int buf, double dt, const double *d_lam_in, d
 int e2d_adjoint_rollout(e2d_ctx *ctx, int nStep0, int nsteps, const double *h_dts, int nsnap,
                         const double *d_lam_n, double *d_lam_0, int64_t *n_forward);
 
+/* Forward half of a checkpointed rollout (no counterpart in the reference,
+ * which has no adjoint): advances the state in buffer nStep0 % 2 by nsteps
+ * frozen-dt steps, as nsteps calls of e2d_godunov_unsplit would, and keeps in
+ * the nsnap snapshots the starting state and the states that the reverse
+ * sweep of e2d_adjoint_rollout stores on its first descent.  The next
+ * e2d_adjoint_rollout call with the same nStep0, nsteps, dts and nsnap, made
+ * with the same starting state back in buffer nStep0 % 2 (checked bit for
+ * bit on the device), starts from them and recomputes that many fewer
+ * forward steps; the gradient is the same bit for bit.  Any other adjoint
+ * rollout call discards the recording. */
+int e2d_rollout_forward(e2d_ctx *ctx, int nStep0, int nsteps, const double *h_dts, int nsnap);
+
 /* FP64 device vector fields in the state's layout (zero-filled on allocation),
  * and their transfer from/to host arrays in the layout of e2d_upload. */
 int e2d_field_alloc(e2d_ctx *ctx, double **d_field);

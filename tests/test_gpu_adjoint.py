@@ -131,6 +131,72 @@ def _record_forward(run, U0, nsteps):
     return np.array(dts)
 
 
+def _first_descent_steps(nsteps, nsnap):
+    """Forward steps of the reverse sweep's first descent (treeverse with
+    Griewank's split), i.e. what a recorded forward saves the sweep."""
+    from math import comb
+    a, s, done = 0, nsnap - 1, 0
+    while nsteps - a > 1 and s > 0:
+        l = nsteps - a
+        t = 1
+        while comb(s + t, s) < l:
+            t += 1
+        m = min(max(int(l - comb(s - 1 + t, s - 1)), 1), l - 1)
+        done, a, s = done + m, a + m, s - 1
+    return done
+
+
+def test_recorded_forward_rollout(gpu_lib):
+    """e2d_rollout_forward: the same final state as the plain steps, the same
+    gradient bit for bit, and the first descent's forward steps saved; a
+    different starting state or dt sequence falls back to the full recompute."""
+    nx, ny, nsteps = 64, 48, 24
+    mask = _interior(nx, ny)
+    rng = np.random.default_rng(5)
+    with e2d.HydroRun(_params(nx, ny)) as run:
+        U0 = _evolved_diag(run, 10)
+        run.init("diag")
+        dts = _record_forward(run, U0, nsteps)
+        run.upload(0, U0)
+        for n in range(nsteps):
+            run.godunov_unsplit(n, dts[n])
+        UT = run.download(nsteps % 2)
+        lam_n = np.where(mask, rng.normal(size=mask.shape), 0.0)
+        d_n, d_0 = run.field_alloc(), run.field_alloc()
+        run.field_upload(d_n, lam_n)
+        for nsnap in (1, 3, 6, nsteps):
+            run.upload(0, U0)
+            nf_plain = run.adjoint_rollout(0, dts, nsnap, d_n, d_0)
+            plain = run.field_download(d_0)
+            run.upload(0, U0)
+            run.rollout_forward(0, dts, nsnap)
+            assert np.array_equal(run.download(nsteps % 2), UT)
+            run.upload(0, U0)
+            nf_rec = run.adjoint_rollout(0, dts, nsnap, d_n, d_0)
+            assert np.array_equal(run.field_download(d_0), plain), f"nsnap={nsnap}"
+            assert nf_rec == nf_plain - _first_descent_steps(nsteps, nsnap), (nsnap, nf_rec, nf_plain)
+            assert np.array_equal(run.download(0)[mask], U0[mask])  # the starting state is back
+            print(f"nsnap={nsnap}: {nf_plain} recomputed steps plain, {nf_rec} after a recorded forward")
+        # a recording made from another starting state, or for other dts, is not used
+        U1 = U0.copy()
+        U1[10, 10, 0] *= 1.0 + 1e-12
+        run.upload(0, U1)
+        nf1 = run.adjoint_rollout(0, dts, 6, d_n, d_0)
+        g1 = run.field_download(d_0)
+        run.upload(0, U0)
+        run.rollout_forward(0, dts, 6)
+        run.upload(0, U1)
+        assert run.adjoint_rollout(0, dts, 6, d_n, d_0) == nf1
+        assert np.array_equal(run.field_download(d_0), g1)
+        run.upload(0, U0)
+        run.rollout_forward(0, dts, 6)
+        run.upload(0, U0)
+        dts2 = list(dts)
+        dts2[-1] *= 0.5
+        nf2 = run.adjoint_rollout(0, dts2, 6, d_n, d_0)
+        assert nf2 == nf1
+
+
 def test_rollout_snapshots_and_tangent_chain(gpu_lib):
     nx, ny, nsteps = 64, 64, 24
     mask = _interior(nx, ny)

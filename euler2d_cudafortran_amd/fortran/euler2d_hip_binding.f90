@@ -134,6 +134,62 @@ module euler2d_hip
        integer(c_int64_t), intent(out) :: cnt(3)
      end function e2d_get_timers
 
+     ! -- discrete adjoint (no counterpart in the reference); device fields are
+     !    FP64 arrays in the state's device layout, passed as type(c_ptr)
+     integer(c_int) function e2d_field_alloc(ctx, d_field) bind(C, name='e2d_field_alloc')
+       import :: c_int, c_ptr
+       type(c_ptr), value       :: ctx
+       type(c_ptr), intent(out) :: d_field
+     end function e2d_field_alloc
+
+     integer(c_int) function e2d_field_free(ctx, d_field) bind(C, name='e2d_field_free')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, d_field
+     end function e2d_field_free
+
+     integer(c_int) function e2d_field_upload(ctx, d_field, h) bind(C, name='e2d_field_upload')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value         :: ctx, d_field
+       real(c_double), intent(in) :: h(*)
+     end function e2d_field_upload
+
+     integer(c_int) function e2d_field_download(ctx, d_field, h) bind(C, name='e2d_field_download')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value          :: ctx, d_field
+       real(c_double), intent(out) :: h(*)
+     end function e2d_field_download
+
+     integer(c_int) function e2d_tangent_step(ctx, buf, dt, d_v_in, d_v_out) bind(C, name='e2d_tangent_step')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value    :: ctx, d_v_in, d_v_out
+       integer(c_int), value :: buf
+       real(c_double), value :: dt
+     end function e2d_tangent_step
+
+     integer(c_int) function e2d_adjoint_step(ctx, buf, dt, d_lam_in, d_lam_out) bind(C, name='e2d_adjoint_step')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value    :: ctx, d_lam_in, d_lam_out
+       integer(c_int), value :: buf
+       real(c_double), value :: dt
+     end function e2d_adjoint_step
+
+     integer(c_int) function e2d_rollout_forward(ctx, nStep0, nsteps, h_dts, nsnap) &
+          bind(C, name='e2d_rollout_forward')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value         :: ctx
+       integer(c_int), value      :: nStep0, nsteps, nsnap
+       real(c_double), intent(in) :: h_dts(*)
+     end function e2d_rollout_forward
+
+     integer(c_int) function e2d_adjoint_rollout(ctx, nStep0, nsteps, h_dts, nsnap, d_lam_n, d_lam_0, &
+          n_forward) bind(C, name='e2d_adjoint_rollout')
+       import :: c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value              :: ctx, d_lam_n, d_lam_0
+       integer(c_int), value           :: nStep0, nsteps, nsnap
+       real(c_double), intent(in)      :: h_dts(*)
+       integer(c_int64_t), intent(out) :: n_forward
+     end function e2d_adjoint_rollout
+
      integer(c_size_t) function c_strlen(s) bind(C, name='strlen')
        import :: c_size_t, c_ptr
        type(c_ptr), value :: s

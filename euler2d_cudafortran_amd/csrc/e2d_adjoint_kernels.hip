@@ -507,6 +507,12 @@ __global__ __launch_bounds__(64 * kAdjWaves) void k_adj_gather_march(const AdjAr
 // and jsize-1).  A part that k_adj_gather skips is carried as -0.0, the
 // identity of IEEE addition, so the sums are bit-identical to the unfused
 // passes.
+// E2D_ADJ_LDS_CARRY=1 (default): the fused kernel's row-carried gather values
+// live in per-thread LDS slots (12 values, 24 KB per FP64 workgroup); in
+// registers the FP64 kernel spills 132 B and is 7% slower (835 vs 778 us).
+#ifndef E2D_ADJ_LDS_CARRY
+#define E2D_ADJ_LDS_CARRY 1
+#endif
 constexpr int kFusedStripOut = 58;
 // Seam buffer: 4 slots of 4 values per completed row and column, indexed by
 // the row (so that the store addresses vary with the marching loop):
@@ -566,15 +572,29 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
   const T dtdx = static_cast<T>(dtdxU), dtdy = static_cast<T>(dtdyU);
   auto row = [&](int r) { return r < 0 ? 0 : (r > g.jsize - 1 ? g.jsize - 1 : r); };
 
-  T qz[4], qa[4], qb[4], qc[4];  // primitives of rows r-2, r-1, r, r+1
+  T qa[4], qb[4], qc[4];  // primitives of rows r-1, r, r+1
   prim_at(a, cl, row(ra - 2), qa);
   prim_at(a, cl, row(ra - 1), qb);
   // row r-1, carried: slopes, trace sources and the bars of its xmin, xmax, ymin face states
   T pdx[4], pdy[4], ps[4], pbxm[4], pbxp[4], pbym[4];
-  // gather, carried: partial sum of the last completed row and its plus part
-  T acc[4], ypf[4];
+  // gather, carried: partial sum of the last completed row, its plus part, and
+  // the primitives of row r-2 (in registers, or with E2D_ADJ_LDS_CARRY in
+  // per-thread LDS slots, out of the registers during the Riemann adjoints)
+#if E2D_ADJ_LDS_CARRY
+  constexpr int kThreads = 64 * kAdjWaves;
+  __shared__ __align__(16) unsigned char carry_raw[12 * sizeof(T) * kThreads];
+  T* const carry = reinterpret_cast<T*>(carry_raw) + threadIdx.x;
+  auto acc = [&](int v) -> T& { return carry[v * kThreads]; };
+  auto ypf = [&](int v) -> T& { return carry[(4 + v) * kThreads]; };
+  auto qz = [&](int v) -> T& { return carry[(8 + v) * kThreads]; };
+#else
+  T acc_r[4], ypf_r[4], qz_r[4];
+  auto acc = [&](int v) -> T& { return acc_r[v]; };
+  auto ypf = [&](int v) -> T& { return ypf_r[v]; };
+  auto qz = [&](int v) -> T& { return qz_r[v]; };
+#endif
 #pragma unroll
-  for (int v = 0; v < 4; ++v) qz[v] = acc[v] = ypf[v] = T(0);
+  for (int v = 0; v < 4; ++v) qz(v) = acc(v) = ypf(v) = T(0);
 
   for (int r = ra - 1; r <= rb; ++r) {
     prim_at(a, cl, row(r + 1), qc);
@@ -649,7 +669,7 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
           }
           const bool yv = limited && done_col && dyb[v] != T(0);
           T ym = T(0), y0 = T(0), yp = T(0);
-          if (yv) limited_slope_adj(st, qz[v], qa[v], qb[v], dyb[v], ym, y0, yp);
+          if (yv) limited_slope_adj(st, qz(v), qa[v], qb[v], dyb[v], ym, y0, yp);
           ymf[v] = yv ? ym : skip;
           y0f[v] = yv ? y0 : skip;
           ypk[v] = yv ? yp : skip;
@@ -663,7 +683,7 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
             // row k-1 is final
             T qfin[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) qfin[v] = acc[v] + ymf[v];
+            for (int v = 0; v < 4; ++v) qfin[v] = acc(v) + ymf[v];
             finish_cell(a, ci, ks - 1, qfin);
           }
           if (k == ra) {
@@ -681,16 +701,16 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
         }
         // partial sum of row k: ((own + x parts) + plus part of row k-1) + own y part
 #pragma unroll
-        for (int v = 0; v < 4; ++v) acc[v] = (qbx[v] + ypf[v]) + y0f[v];
+        for (int v = 0; v < 4; ++v) acc(v) = (qbx[v] + ypf(v)) + y0f[v];
         if (gat_col && k == rb - 1) {
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            if (k > ra) a.seam[seam_at(g, ks, 1, v, ci)] = acc[v];
+            if (k > ra) a.seam[seam_at(g, ks, 1, v, ci)] = acc(v);
             a.seam[seam_at(g, ks, 3, v, ci)] = ypk[v];
           }
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) ypf[v] = ypk[v];
+        for (int v = 0; v < 4; ++v) ypf(v) = ypk[v];
       }
       // x-faces of row r
       if (r < rb) {
@@ -724,7 +744,7 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
       pbxm[v] = bxm[v];
       pbxp[v] = bxp[v];
       pbym[v] = bym[v];
-      qz[v] = qa[v];
+      qz(v) = qa[v];
       qa[v] = qb[v];
       qb[v] = qc[v];
     }
@@ -889,16 +909,14 @@ __global__ __launch_bounds__(256) void k_adj_fold_frame(const AdjArgs<P> a) {
 //           through HBM)
 //   separate  k_adj_prim, k_adj_faces, k_adj_cells, k_adj_gather (face bars
 //           through HBM as well)
-// E2D_ADJOINT_PASSES=fused|march|separate chooses; by default FP32 arithmetic
-// (fp32 and mixed) runs fused and FP64 runs march: the fused FP64 kernel needs
-// more than 256 VGPRs and its spills make it slower (2048^2, dense lam: FP64
-// 841 us fused, 812 us march; FP32 487 us fused, 535 us march).  The fused
-// kernel writes lam_out while other wavefronts still read lam_in, so an
-// aliased call (lam_out == lam_in) runs march, whose gather reads and writes
-// each element of lam in the same thread.
+// E2D_ADJOINT_PASSES=fused|march|separate chooses; fused is the default
+// (2048^2, dense lam: FP64 778 us fused, 796 us march; FP32 477 us fused,
+// 527 us march).  The fused kernel writes lam_out while other wavefronts still
+// read lam_in, so an aliased call (lam_out == lam_in) runs march, whose gather
+// reads and writes each element of lam in the same thread.
 enum class Passes { kFused, kMarch, kSeparate };
-Passes adjoint_passes(bool fp64, bool aliased) {
-  Passes p = fp64 ? Passes::kMarch : Passes::kFused;
+Passes adjoint_passes(bool aliased) {
+  Passes p = Passes::kFused;
   if (const char* env = getenv("E2D_ADJOINT_PASSES")) {
     if (strcmp(env, "separate") == 0) p = Passes::kSeparate;
     if (strcmp(env, "march") == 0) p = Passes::kMarch;
@@ -957,7 +975,7 @@ hipError_t adjoint_step(const e2d_params& p, const DevBuffers& b, int buf, doubl
   hipError_t e = launch_boundaries(p, b, buf, st);
   if (e != hipSuccess) return e;
   const Grid& g = a.g;
-  const Passes passes = adjoint_passes(sizeof(T) == sizeof(double), lam_in == lam_out);
+  const Passes passes = adjoint_passes(lam_in == lam_out);
   if (passes == Passes::kSeparate) {
     hipLaunchKernelGGL(k_adj_prim<P>, dim3((g.isize + 63) / 64, (g.jsize + 3) / 4), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_adj_faces<P>, dim3((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4), dim3(256), 0, st, a);

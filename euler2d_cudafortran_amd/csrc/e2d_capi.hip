@@ -220,6 +220,39 @@ void free_all(e2d_ctx* c) {
   if (c->stream) (void)hipStreamDestroy(c->stream), c->stream = nullptr;
 }
 
+// Binomial checkpointing (Griewank's treeverse).  beta(s, t) = C(s+t, s) steps
+// can be reversed with s snapshots and at most t forward sweeps per step.
+double beta(int s, int t) {
+  double r = 1.0;
+  for (int k = 1; k <= s; ++k) r = r * (t + k) / k;
+  return r;
+}
+
+// (step, slot) of the snapshots that Rollout::reverse stores on its first
+// descent from (0, nsteps) with nsnap slots: the same split and slot order
+std::vector<std::pair<int, int>> first_descent(int nsteps, int nsnap) {
+  std::vector<std::pair<int, int>> out;
+  std::vector<int> free_slots;
+  for (int k = nsnap - 1; k >= 1; --k) free_slots.push_back(k);
+  int a = 0;
+  const int b = nsteps;
+  for (;;) {
+    const int l = b - a;
+    const int s = static_cast<int>(free_slots.size());
+    if (l == 1 || s == 0) break;
+    int t = 1;
+    while (beta(s, t) < l) ++t;
+    int m = static_cast<int>(l - beta(s - 1, t));
+    if (m < 1) m = 1;
+    if (m > l - 1) m = l - 1;
+    const int mine = free_slots.back();
+    free_slots.pop_back();
+    out.emplace_back(a + m, mine);
+    a += m;
+  }
+  return out;
+}
+
 }  // namespace
 
 extern "C" {
@@ -557,39 +590,6 @@ int forward_step(e2d_ctx* c, int n, double dt) {
   cfg.dt_host = dt;
   HIP_TRY(e2d::launch_fused_step(c->p, c->b, cfg, c->stream));
   return E2D_OK;
-}
-
-// Binomial checkpointing (Griewank's treeverse).  beta(s, t) = C(s+t, s) steps
-// can be reversed with s snapshots and at most t forward sweeps per step.
-double beta(int s, int t) {
-  double r = 1.0;
-  for (int k = 1; k <= s; ++k) r = r * (t + k) / k;
-  return r;
-}
-
-// (step, slot) of the snapshots that Rollout::reverse stores on its first
-// descent from (0, nsteps) with nsnap slots: the same split and slot order
-std::vector<std::pair<int, int>> first_descent(int nsteps, int nsnap) {
-  std::vector<std::pair<int, int>> out;
-  std::vector<int> free_slots;
-  for (int k = nsnap - 1; k >= 1; --k) free_slots.push_back(k);
-  int a = 0;
-  const int b = nsteps;
-  for (;;) {
-    const int l = b - a;
-    const int s = static_cast<int>(free_slots.size());
-    if (l == 1 || s == 0) break;
-    int t = 1;
-    while (beta(s, t) < l) ++t;
-    int m = static_cast<int>(l - beta(s - 1, t));
-    if (m < 1) m = 1;
-    if (m > l - 1) m = l - 1;
-    const int mine = free_slots.back();
-    free_slots.pop_back();
-    out.emplace_back(a + m, mine);
-    a += m;
-  }
-  return out;
 }
 
 struct Rollout {

@@ -75,7 +75,7 @@ struct e2d_ctx {
   double* staging = nullptr;  // float64 host-layout staging array on the device
   double* d_sums = nullptr;
   // running invDt of compute_dt: authoritative on the host when dev_slot < 0,
-  // else in b.scal[dev_slot]
+  // else in device slot dev_slot of b.scal (the maximum of its parts)
   double invdt = 0.0;
   int dev_slot = -1;
   bool stages_valid = false;
@@ -122,11 +122,28 @@ int check_buf(int buf) {
 
 size_t host_elems(const e2d_ctx* c) { return static_cast<size_t>(c->isize) * c->jsize * 4; }
 
+// set every part of running-invDt slot `slot` to v
+int fill_slot(e2d_ctx* c, int slot, double v) {
+  HIP_TRY(e2d::launch_fill(c->b.scal + slot * e2d::kInvdtSlot, e2d::kInvdtSlot, v, c->stream));
+  return E2D_OK;
+}
+
+// value of running-invDt slot `slot` (the maximum of its parts); synchronous
+int read_slot(e2d_ctx* c, int slot, double* v) {
+  double parts[e2d::kInvdtSlot];
+  HIP_TRY(hipMemcpyAsync(parts, c->b.scal + slot * e2d::kInvdtSlot, sizeof(parts), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  double m = parts[0];
+  for (int p = 1; p < e2d::kInvdtParts; ++p) m = m > parts[p * e2d::kInvdtStride] ? m : parts[p * e2d::kInvdtStride];
+  *v = m;
+  return E2D_OK;
+}
+
 // bring the running invDt to the host
 int pull_invdt(e2d_ctx* c) {
   if (c->dev_slot >= 0) {
-    HIP_TRY(hipMemcpyAsync(&c->invdt, c->b.scal + c->dev_slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (int rc = read_slot(c, c->dev_slot, &c->invdt)) return rc;
     c->dev_slot = -1;
   }
   return E2D_OK;
@@ -292,7 +309,8 @@ int e2d_create(const e2d_params* p, e2d_ctx** out) {
   const size_t n = static_cast<size_t>(c->b.plane) * 4;
   const size_t es = e2d::state_elem_bytes(p->precision);
   bool ok = hipMalloc(&c->b.u[0], n * es) == hipSuccess && hipMalloc(&c->b.u[1], n * es) == hipSuccess &&
-            hipMalloc(reinterpret_cast<void**>(&c->b.scal), 4 * sizeof(double)) == hipSuccess &&
+            hipMalloc(reinterpret_cast<void**>(&c->b.scal), e2d::kInvdtSlots * e2d::kInvdtSlot * sizeof(double)) ==
+                hipSuccess &&
             hipMalloc(reinterpret_cast<void**>(&c->b.partial), e2d::kSumBlocks * 4 * sizeof(double)) == hipSuccess &&
             hipMalloc(reinterpret_cast<void**>(&c->d_sums), 4 * sizeof(double)) == hipSuccess;
   if (ok && p->implementation_version != E2D_IMPL_FUSED) {
@@ -312,7 +330,7 @@ int e2d_create(const e2d_params* p, e2d_ctx** out) {
   // both buffers start at zero (u2 as the reference leaves it), padding included
   if (hipMemsetAsync(c->b.u[0], 0, n * es, c->stream) != hipSuccess ||
       hipMemsetAsync(c->b.u[1], 0, n * es, c->stream) != hipSuccess ||
-      e2d::launch_fill(c->b.scal, 4, 0.0, c->stream) != hipSuccess ||
+      e2d::launch_fill(c->b.scal, e2d::kInvdtSlots * e2d::kInvdtSlot, 0.0, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess) {
     free_all(c);
     delete c;
@@ -395,10 +413,9 @@ int e2d_compute_dt(e2d_ctx* c, int useU, double* dt) {
   if (!dt) return fail(E2D_ERR_ARG, "dt is NULL");
   if (int rc = pull_invdt(c)) return rc;
   const int buf = (useU == 0) ? 0 : 1;
-  HIP_TRY(e2d::launch_fill(c->b.scal + 2, 1, c->invdt, c->stream));
+  if (int rc = fill_slot(c, 2, c->invdt)) return rc;
   HIP_TRY(timed(c, E2D_TIMER_DT, [&] { return e2d::launch_max_invdt(c->p, c->b, buf, 2, c->stream); }));
-  HIP_TRY(hipMemcpyAsync(&c->invdt, c->b.scal + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (int rc = read_slot(c, 2, &c->invdt)) return rc;
   *dt = dt_of(c->p, c->invdt);
   return E2D_OK;
 }
@@ -442,7 +459,7 @@ int e2d_run_steps(e2d_ctx* c, int nsteps, int* nStep) {
   if (k < 0) {
     // compute_dt of the current input, folded into the running maximum
     k = 0;
-    HIP_TRY(e2d::launch_fill(c->b.scal + k, 1, c->invdt, c->stream));
+    if (int rc = fill_slot(c, k, c->invdt)) return rc;
     HIP_TRY(timed(c, E2D_TIMER_DT, [&] { return e2d::launch_max_invdt(c->p, c->b, n % 2, k, c->stream); }));
   }
   e2d::StepCfg cfg;
@@ -450,7 +467,7 @@ int e2d_run_steps(e2d_ctx* c, int nsteps, int* nStep) {
   if (c->p.implementation_version == E2D_IMPL_FUSED) {
     // slot k holds the running invDt including the current input; the other
     // slot must not exceed it before the first step writes there
-    HIP_TRY(e2d::launch_fill(c->b.scal + (1 - k), 1, 0.0, c->stream));
+    if (int rc = fill_slot(c, 1 - k, 0.0)) return rc;
     for (int s = 0; s < nsteps; ++s, ++n) {
       cfg.buf_in = n % 2;
       cfg.buf_out = 1 - cfg.buf_in;

@@ -192,6 +192,23 @@ __device__ __forceinline__ void atomic_max_nonneg(double* dst, double v) {
   atomicMax(reinterpret_cast<unsigned long long*>(dst), static_cast<unsigned long long>(__double_as_longlong(v)));
 }
 
+// value of running-invDt slot `slot`: the maximum of its parts (uniform loads)
+__device__ __forceinline__ double invdt_read(const double* scal, int slot) {
+  const double* s = scal + slot * kInvdtSlot;
+  double m = s[0];
+#pragma unroll
+  for (int p = 1; p < kInvdtParts; ++p) {
+    const double x = s[p * kInvdtStride];
+    m = m > x ? m : x;
+  }
+  return m;
+}
+
+// fold a non-negative rate into slot `slot`, part chosen by `who`
+__device__ __forceinline__ void invdt_fold(double* scal, int slot, int who, double v) {
+  atomic_max_nonneg(scal + slot * kInvdtSlot + (who % kInvdtParts) * kInvdtStride, v);
+}
+
 // XCD-aware block order: hardware hands consecutive workgroups to the 8 XCDs
 // in turn; this gives each XCD a contiguous range of logical blocks so that
 // neighbouring strips/segments (which share halo rows and columns) meet in the

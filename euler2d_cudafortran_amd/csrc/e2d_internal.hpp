@@ -11,6 +11,16 @@ namespace e2d {
 
 constexpr int kNumStages = 9;  // E2D_STAGE_Q .. E2D_STAGE_QYP
 
+// Device running-invDt slots.  Slot s holds kInvdtParts partial maxima, one
+// per 128-byte line, and its value is their maximum: the atomic maxima that
+// every wavefront of a step folds in spread over independent lines instead of
+// serialising on one address (at 2048^2 a step folds 6,000 of them).  Slots 0
+// and 1 ping-pong between steps; slot 2 serves compute_dt.
+constexpr int kInvdtParts = 32;
+constexpr int kInvdtStride = 16;                          // doubles between parts
+constexpr int kInvdtSlot = kInvdtParts * kInvdtStride;    // doubles per slot
+constexpr int kInvdtSlots = 3;
+
 // Device memory of one run.  State element (i, j, v), 0-based, of buffer b
 // sits at u[b][v * plane + j * pitch + i].
 struct DevBuffers {
@@ -18,9 +28,7 @@ struct DevBuffers {
   int64_t pitch = 0;  // elements per row (>= isize, padded)
   int64_t plane = 0;  // elements per variable plane (pitch * jsize)
   void* stage[kNumStages] = {};
-  // device scalars: [0], [1] running-invDt slots (ping-pong), [2] reduction
-  // target of a standalone compute_dt
-  double* scal = nullptr;
+  double* scal = nullptr;  // running-invDt slots (kInvdtSlots x kInvdtSlot doubles)
   double* partial = nullptr;  // per-block partial sums (kSumBlocks * 4)
 };
 
@@ -29,10 +37,10 @@ constexpr int kSumBlocks = 240;
 // How a step obtains dt.
 struct StepCfg {
   int buf_in = 0, buf_out = 1;
-  bool dt_from_device = false;  // dt = cfl / scal[slot_cur]; else dt_host
+  bool dt_from_device = false;  // dt = cfl / (invDt slot slot_cur); else dt_host
   double dt_host = 0.0;
   int slot_cur = 0;
-  bool track_dt = false;  // fold the new state's CFL rate into scal[slot_next]
+  bool track_dt = false;  // fold the new state's CFL rate into invDt slot slot_next
   int slot_next = 1;
 };
 
@@ -44,7 +52,7 @@ hipError_t launch_fused_step(const e2d_params& p, const DevBuffers& b, const Ste
 hipError_t launch_materialised_step(const e2d_params& p, const DevBuffers& b, const StepCfg& c,
                                     hipStream_t s);
 hipError_t launch_boundaries(const e2d_params& p, const DevBuffers& b, int buf, hipStream_t s);
-// atomic max of the CFL rate over the compute_dt window of buffer buf into scal[slot]
+// atomic max of the CFL rate over the compute_dt window of buffer buf into invDt slot `slot`
 hipError_t launch_max_invdt(const e2d_params& p, const DevBuffers& b, int buf, int slot, hipStream_t s);
 hipError_t launch_fill(double* dst, int n, double v, hipStream_t s);
 // *flag (device) = 1 when the two device arrays of `bytes` bytes differ in any bit, else 0

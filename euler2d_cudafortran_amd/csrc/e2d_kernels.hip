@@ -121,7 +121,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) E2D_FUSED_ATTR void k_step_fus
   U dt;
   double invdt_cur = 0.0;
   if (a.dt_from_device) {
-    invdt_cur = a.scal[a.slot_cur];
+    invdt_cur = invdt_read(a.scal, a.slot_cur);
     dt = a.hU.cfl / static_cast<U>(invdt_cur);
   } else {
     dt = static_cast<U>(a.dt_host);
@@ -228,17 +228,19 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) E2D_FUSED_ATTR void k_step_fus
     double m = static_cast<double>(rate_max);
     m = m > invdt_cur ? m : invdt_cur;  // running maximum carries over
     m = wave_max(m);
-    if (lane == 0) atomic_max_nonneg(a.scal + a.slot_next, m);
+    if (lane == 0) invdt_fold(a.scal, a.slot_next, lb * kWavesPerBlock + wave, m);
   }
 }
 
 // Rows per wavefront segment.  Each segment re-primes its rolling rows (two
 // extra row loads and primitive conversions, one extra row of y-faces), so
 // long segments are cheaper per row; short segments give more wavefronts to
-// hide latency and balance the data-dependent Newton work.  Measured on
-// MI355X (FP64, 3 resident waves per SIMD): about ny/170 rows is best
-// (12 at 2048^2, 24 at 4096^2); kernels with more resident waves per SIMD
-// want proportionally fewer rows.  E2D_FUSED_SEG_ROWS overrides the choice.
+// hide latency and balance the data-dependent Newton work, and on small grids
+// a wavefront's rows are the critical path.  Measured on MI355X (FP64, 3
+// resident waves per SIMD): about ny/170 rows is best at every size, 1 at
+// 128^2 (16.7 us/step against 44 us with 8 rows), 2 at 256^2, 3-6 at 512^2
+// and 1024^2, 12 at 2048^2, 24 at 4096^2; kernels with more resident waves
+// per SIMD want proportionally fewer rows.  E2D_FUSED_SEG_ROWS overrides it.
 static int choose_seg_rows(const void* kernel, int ny) {
   if (const char* env = getenv("E2D_FUSED_SEG_ROWS")) {
     const int r = atoi(env);
@@ -251,7 +253,7 @@ static int choose_seg_rows(const void* kernel, int ny) {
     blocks_per_cu = 12 / kWavesPerBlock;
   const double waves_per_simd = blocks_per_cu * kWavesPerBlock / 4.0;
   const int r = static_cast<int>(ny / 170.0 * 3.0 / waves_per_simd + 0.5);
-  return r < 8 ? 8 : (r > 32 ? 32 : r);
+  return r < 1 ? 1 : (r > 32 ? 32 : r);
 }
 
 // per host thread, so that contexts driven from different threads do not race
@@ -416,7 +418,7 @@ hipError_t launch_boundaries(const e2d_params& p, const DevBuffers& b, int buf, 
 
 template <class P>
 __global__ __launch_bounds__(256) void k_max_invdt(const Hydro<typename P::U> h, const Grid g,
-                                                    const typename P::S* __restrict__ u, double* dst) {
+                                                    const typename P::S* __restrict__ u, double* scal, int slot) {
   using U = typename P::U;
   __shared__ double red[4];
   const int w = g.nx - 1, hgt = g.ny - 1;  // window extent
@@ -438,7 +440,7 @@ __global__ __launch_bounds__(256) void k_max_invdt(const Hydro<typename P::U> h,
   if (threadIdx.x == 0) {
     double x = red[0];
     for (int k = 1; k < 4; ++k) x = x > red[k] ? x : red[k];
-    atomic_max_nonneg(dst, x);
+    invdt_fold(scal, slot, blockIdx.x, x);
   }
 }
 
@@ -450,7 +452,7 @@ static hipError_t max_invdt(const e2d_params& p, const DevBuffers& b, int buf, i
   if (nb > 2048) nb = 2048;
   if (nb < 1) nb = 1;
   hipLaunchKernelGGL(k_max_invdt<P>, dim3(nb), dim3(256), 0, st, make_hydro<typename P::U>(p), g,
-                     static_cast<const typename P::S*>(b.u[buf]), b.scal + slot);
+                     static_cast<const typename P::S*>(b.u[buf]), b.scal, slot);
   return hipGetLastError();
 }
 
@@ -512,7 +514,7 @@ struct MatArgs {
 template <class P>
 __device__ __forceinline__ void step_dt(const MatArgs<P>& a, typename P::U& dtdx, typename P::U& dtdy) {
   using U = typename P::U;
-  const U dt = a.dt_from_device ? a.hU.cfl / static_cast<U>(a.scal[a.slot_cur]) : static_cast<U>(a.dt_host);
+  const U dt = a.dt_from_device ? a.hU.cfl / static_cast<U>(invdt_read(a.scal, a.slot_cur)) : static_cast<U>(a.dt_host);
   dtdx = dt / a.hU.dx;
   dtdy = dt / a.hU.dy;
 }

@@ -73,6 +73,9 @@ E2D_HDI Hydro<Dual> dual_hydro(const Hydro<T>& h) {
   r.entho = h.entho;
   r.dx = h.dx;
   r.dy = h.dy;
+  r.inv_dx = h.inv_dx;
+  r.inv_dy = h.inv_dy;
+  r.dxy_pow2 = h.dxy_pow2;
   r.conv_tol = h.conv_tol;
   r.slope_type = h.slope_type;
   r.niter = h.niter;

@@ -43,6 +43,9 @@ inline Hydro<T> make_hydro(const e2d_params& p) {
   h.entho = T(1) / (h.gamma0 - T(1));
   h.dx = (T(1) - T(0)) / static_cast<T>(p.nx);
   h.dy = (T(1) - T(0)) / static_cast<T>(p.ny);
+  h.dxy_pow2 = p.nx > 0 && p.ny > 0 && (p.nx & (p.nx - 1)) == 0 && (p.ny & (p.ny - 1)) == 0;
+  h.inv_dx = static_cast<T>(p.nx);
+  h.inv_dy = static_cast<T>(p.ny);
   h.conv_tol = static_cast<T>(1e-6f);
   h.slope_type = p.slope_type;
   h.niter = p.niter_riemann;

@@ -33,6 +33,11 @@ struct Hydro {
   T gamma0, gamma6, cfl, smallr, smallc, smallp, smallpp;
   T entho;     // 1/(gamma0-1), evaluated as the reference evaluates it in cmpflx
   T dx, dy;
+  // 1/dx and 1/dy, used when both cell sizes are powers of two (nx and ny
+  // powers of two): then x / dx equals x * (1/dx) exactly for every x, the
+  // exact quotient being representable whenever the division does not overflow
+  T inv_dx, inv_dy;
+  int dxy_pow2;
   T conv_tol;  // Newton stop threshold: the FP32 literal 1e-6 widened to T (m_utils.f90:396)
   int slope_type, niter, riemann_solver;
 };
@@ -176,6 +181,7 @@ E2D_HDI T cfl_rate(const Hydro<T>& h, const T u[4]) {
   const T c = sound_speed<T, M>(h, q);
   const T ax = c + fabs_(q[2]);
   const T ay = c + fabs_(q[3]);
+  if (h.dxy_pow2) return ax * h.inv_dx + ay * h.inv_dy;
   return M::div(ax, h.dx) + M::div(ay, h.dy);
 }
 

@@ -28,6 +28,8 @@ static Hydro<double> hydro(int slope_type) {
   h.gamma6 = (h.gamma0 + 1.0) / (2.0 * h.gamma0);
   h.entho = 1.0 / (h.gamma0 - 1.0);
   h.dx = h.dy = 1.0 / 64;
+  h.inv_dx = h.inv_dy = 64.0;
+  h.dxy_pow2 = 1;
   h.conv_tol = static_cast<double>(1e-6f);
   h.slope_type = slope_type;
   h.niter = 10;
@@ -218,6 +220,8 @@ int main() {
     hf.smallpp = static_cast<float>(h.smallpp);
     hf.entho = static_cast<float>(h.entho);
     hf.dx = hf.dy = static_cast<float>(h.dx);
+    hf.inv_dx = hf.inv_dy = static_cast<float>(h.inv_dx);
+    hf.dxy_pow2 = h.dxy_pow2;
     hf.conv_tol = 1e-6f;
     hf.slope_type = h.slope_type;
     hf.niter = h.niter;

@@ -309,7 +309,7 @@ int e2d_create(const e2d_params* p, e2d_ctx** out) {
   const size_t n = static_cast<size_t>(c->b.plane) * 4;
   const size_t es = e2d::state_elem_bytes(p->precision);
   bool ok = hipMalloc(&c->b.u[0], n * es) == hipSuccess && hipMalloc(&c->b.u[1], n * es) == hipSuccess &&
-            hipMalloc(reinterpret_cast<void**>(&c->b.scal), e2d::kInvdtSlots * e2d::kInvdtSlot * sizeof(double)) ==
+            hipMalloc(reinterpret_cast<void**>(&c->b.scal), e2d::kScalDoubles * sizeof(double)) ==
                 hipSuccess &&
             hipMalloc(reinterpret_cast<void**>(&c->b.partial), e2d::kSumBlocks * 4 * sizeof(double)) == hipSuccess &&
             hipMalloc(reinterpret_cast<void**>(&c->d_sums), 4 * sizeof(double)) == hipSuccess;
@@ -330,7 +330,7 @@ int e2d_create(const e2d_params* p, e2d_ctx** out) {
   // both buffers start at zero (u2 as the reference leaves it), padding included
   if (hipMemsetAsync(c->b.u[0], 0, n * es, c->stream) != hipSuccess ||
       hipMemsetAsync(c->b.u[1], 0, n * es, c->stream) != hipSuccess ||
-      e2d::launch_fill(c->b.scal, e2d::kInvdtSlots * e2d::kInvdtSlot, 0.0, c->stream) != hipSuccess ||
+      e2d::launch_fill(c->b.scal, e2d::kScalDoubles, 0.0, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess) {
     free_all(c);
     delete c;

@@ -20,6 +20,7 @@ constexpr int kInvdtParts = 32;
 constexpr int kInvdtStride = 16;                          // doubles between parts
 constexpr int kInvdtSlot = kInvdtParts * kInvdtStride;    // doubles per slot
 constexpr int kInvdtSlots = 3;
+constexpr int kScalDoubles = kInvdtSlots * kInvdtSlot;  // doubles in the allocation
 
 // Device memory of one run.  State element (i, j, v), 0-based, of buffer b
 // sits at u[b][v * plane + j * pitch + i].

@@ -20,6 +20,7 @@
 // serial_ref arithmetic).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
 
@@ -56,10 +57,46 @@ struct StepArgs {
   double* scal;
   int dt_from_device, slot_cur, track_dt, slot_next;
   double dt_host;
-  int nsegs;      // row segments per strip
-  int seg_rows;   // output rows per segment
-  int seg_groups; // ceil(nsegs / kWavesPerBlock)
+  // Row segments, one per wavefront: items [0, nlong) are nsegs segments of
+  // seg_rows rows per strip (strip-major) over rows [2, short_base); items
+  // [nlong, nitems) are nshort segments of short_rows rows per strip over
+  // [short_base, ny + 2).
+  int nsegs, seg_rows, nlong;
+  int nshort, short_rows, short_base, nitems;
 };
+
+// Segment (strip, rows [j0, j1)) of wavefront `wave` of block b; false for a
+// wavefront without one.  Blocks go to the 8 XCDs round-robin (b % 8) and
+// start in order on each: XCD x takes the x-th eighth of the long segments,
+// then the x-th eighth of the short ones.  Every XCD thus ends its share on
+// short segments, which shortens the tail where the device runs out of
+// wavefronts, and the segments that meet in one L2 are neighbours.
+template <class P>
+__device__ __forceinline__ bool fused_segment(const StepArgs<P>& a, int b, int wave, int& strip, int& j0,
+                                              int& j1) {
+  const int x = b % 8;
+  const int k = b / 8 * kWavesPerBlock + wave;
+  const int ns = a.nitems - a.nlong;
+  const int l0 = static_cast<int>(static_cast<int64_t>(a.nlong) * x / 8);
+  const int nl = static_cast<int>(static_cast<int64_t>(a.nlong) * (x + 1) / 8) - l0;
+  const int s0 = static_cast<int>(static_cast<int64_t>(ns) * x / 8);
+  const int nsx = static_cast<int>(static_cast<int64_t>(ns) * (x + 1) / 8) - s0;
+  if (k < nl) {
+    const int item = l0 + k;
+    strip = item / a.nsegs;
+    j0 = 2 + (item - strip * a.nsegs) * a.seg_rows;
+    j1 = min(j0 + a.seg_rows, a.short_base);
+    return true;
+  }
+  if (k < nl + nsx) {
+    const int item = s0 + (k - nl);
+    strip = item / a.nshort;
+    j0 = a.short_base + (item - strip * a.nshort) * a.short_rows;
+    j1 = min(j0 + a.short_rows, a.g.ny + 2);
+    return true;
+  }
+  return false;
+}
 
 // Load cell (ci, r) of the input, boundary-filled (storage type); write the
 // ghost value back so the input buffer ends up as make_boundaries leaves it.
@@ -89,9 +126,6 @@ __device__ __forceinline__ void prim_of(const Hydro<typename P::T>& h, const typ
   cons_to_prim<T, typename P::M>(h, u, q);
 }
 
-#ifndef E2D_BAND_FWD
-#define E2D_BAND_FWD 0
-#endif
 // kMinmod: instantiated for slope_type == 1 (the benchmark's minmod), whose
 // limiter then skips the products by st = 1 and the first-order select
 template <class P, int SOLVER, bool kMinmod = false>
@@ -101,27 +135,21 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) E2D_FUSED_ATTR void k_step_fus
   using S = typename P::S;
   using M = typename P::M;
 
-  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  int strip, group;
-  strip_and_group<E2D_BAND_FWD>(lb, a.seg_groups, strip, group);
-  const int seg = group * kWavesPerBlock + wave;
-  if (seg >= a.nsegs) return;  // whole wavefront leaves: no barrier in this kernel
-
   const Grid& g = a.g;
+  int strip, j0, j1;
+  if (!fused_segment(a, blockIdx.x, wave, strip, j0, j1)) return;  // whole wavefront: no barrier here
   const int ci = strip * kStripOut + lane;  // column of this lane
   const bool col_ok = ci < g.isize;
   const int cl = col_ok ? ci : g.isize - 1;  // clamped column for loads
   const bool out_lane = (lane >= 2) && (lane < 62) && (ci <= g.nx + 1);
-  const int j0 = 2 + seg * a.seg_rows;
-  const int j1 = min(j0 + a.seg_rows, g.ny + 2);
 
   // dt of this step (serial_ref/m_run.f90:308, 384-385)
   U dt;
   double invdt_cur = 0.0;
   if (a.dt_from_device) {
-    invdt_cur = invdt_read(a.scal, a.slot_cur);
+    invdt_cur = invdt_read_wave(a.scal, a.slot_cur);
     dt = a.hU.cfl / static_cast<U>(invdt_cur);
   } else {
     dt = static_cast<U>(a.dt_host);
@@ -131,6 +159,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) E2D_FUSED_ATTR void k_step_fus
   const T dtdx = static_cast<T>(dtdxU);
   const T dtdy = static_cast<T>(dtdyU);
   const Hydro<T>& h = a.hT;
+  U rate_max = U(0);
 
   T qa[4], qb[4], qc[4];
   S raw[4], raw_next[4];
@@ -143,7 +172,6 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) E2D_FUSED_ATTR void k_step_fus
   T ymax_prev[4];
   U fy_prev[4] = {U(0), U(0), U(0), U(0)}, fx_prev[4] = {U(0), U(0), U(0), U(0)};
   U fy_cur[4] = {U(0), U(0), U(0), U(0)}, fx_cur[4] = {U(0), U(0), U(0), U(0)};
-  U rate_max = U(0);
 
   for (int r = j0 - 1; r <= j1; ++r) {
     // software pipeline: row r+1 was fetched one iteration ago; fetch row r+2
@@ -228,43 +256,77 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) E2D_FUSED_ATTR void k_step_fus
     double m = static_cast<double>(rate_max);
     m = m > invdt_cur ? m : invdt_cur;  // running maximum carries over
     m = wave_max(m);
-    if (lane == 0) invdt_fold(a.scal, a.slot_next, lb * kWavesPerBlock + wave, m);
+    if (lane == 0) invdt_fold(a.scal, a.slot_next, static_cast<int>(blockIdx.x) * kWavesPerBlock + wave, m);
   }
 }
 
-// Rows per wavefront segment.  Each segment re-primes its rolling rows (two
-// extra row loads and primitive conversions, one extra row of y-faces), so
-// long segments are cheaper per row; short segments give more wavefronts to
-// hide latency and balance the data-dependent Newton work, and on small grids
-// a wavefront's rows are the critical path.  Measured on MI355X (FP64, 3
-// resident waves per SIMD): about ny/170 rows is best at every size, 1 at
-// 128^2 (16.7 us/step against 44 us with 8 rows), 2 at 256^2, 3-6 at 512^2
-// and 1024^2, 12 at 2048^2, 24 at 4096^2; kernels with more resident waves
-// per SIMD want proportionally fewer rows.  E2D_FUSED_SEG_ROWS overrides it.
-static int choose_seg_rows(const void* kernel, int ny) {
-  if (const char* env = getenv("E2D_FUSED_SEG_ROWS")) {
-    const int r = atoi(env);
-    if (r >= 1) return r;
-  }
+static int env_int(const char* name, int dflt) {
+  const char* v = getenv(name);
+  return v && *v ? atoi(v) : dflt;
+}
+
+// Rows per long segment.  Each segment re-primes its rolling rows (two extra
+// row loads and primitive conversions, one extra row of y-faces), so long
+// segments are cheaper per row; short ones give more wavefronts to hide
+// latency and balance the data-dependent Newton work, and on small grids a
+// wavefront's rows are the critical path.  About ny/128 rows at 3 resident
+// wavefronts per SIMD (the FP64 kernels), proportionally fewer where more
+// are resident, within [1, 32]; E2D_FUSED_SEG_ROWS overrides it.  Measured on
+// MI355X with the 75/25 split below: 16 rows at 2048^2 (FP64 228-234 us/step
+// against 251 us for uniform ny/170-row segments on the same device), 32 at
+// 4096^2, 8 at 1024^2.
+static int fused_seg_rows(int ny, int waves_per_simd = 3) {
+  const int r = env_int("E2D_FUSED_SEG_ROWS", 0);
+  if (r >= 1) return r;
+  const int d = static_cast<int>(ny / 128.0 * 3.0 / std::max(1, waves_per_simd) + 0.5);
+  return d < 1 ? 1 : (d > 32 ? 32 : d);
+}
+
+// resident wavefronts per SIMD of a kernel launched with kWavesPerBlock waves per block
+static int waves_per_simd(const void* kernel) {
   int blocks_per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, kernel, 64 * kWavesPerBlock, 0) !=
           hipSuccess ||
       blocks_per_cu <= 0)
-    blocks_per_cu = 12 / kWavesPerBlock;
-  const double waves_per_simd = blocks_per_cu * kWavesPerBlock / 4.0;
-  const int r = static_cast<int>(ny / 170.0 * 3.0 / waves_per_simd + 0.5);
-  return r < 1 ? 1 : (r > 32 ? 32 : r);
+    return 3;
+  return std::max(1, blocks_per_cu * kWavesPerBlock / 4);
 }
 
-// per host thread, so that contexts driven from different threads do not race
-template <class P, int SOLVER, bool kMinmod = false>
-static int seg_rows_for(int ny) {
-  thread_local int cached_ny = -1, cached = 16;
-  if (ny != cached_ny) {
-    cached = choose_seg_rows(reinterpret_cast<const void*>(k_step_fused<P, SOLVER, kMinmod>), ny);
-    cached_ny = ny;
+// Segment plan of a fused launch over nstrips strips: long segments over the
+// lower 75% of each strip's rows (E2D_FUSED_LONG_PCT), short ones of a quarter
+// of their length over the rest (E2D_FUSED_SHORT_ROWS); the environment is
+// read at every launch.  Returns the number of blocks.
+template <class P>
+static int plan_segments(StepArgs<P>& a, int nstrips, int ny, int wps) {
+  const int R = fused_seg_rows(ny, wps);
+  const int r = std::max(1, env_int("E2D_FUSED_SHORT_ROWS", R / 4));
+  const int pct = std::min(100, std::max(0, env_int("E2D_FUSED_LONG_PCT", 75)));
+  const int L = pct >= 100 ? (ny + R - 1) / R : static_cast<int>(static_cast<int64_t>(ny) * pct / 100 / R);
+  const int covered = std::min(L * R, ny);
+  a.seg_rows = R;
+  a.nsegs = L;
+  a.nlong = nstrips * L;
+  a.short_base = 2 + covered;
+  a.short_rows = r;
+  a.nshort = covered < ny ? (ny - covered + r - 1) / r : 0;
+  a.nitems = a.nlong + nstrips * a.nshort;
+  // enough blocks for the XCD with the most segments (see fused_segment)
+  const int ns = a.nitems - a.nlong;
+  int most = 0;
+  for (int x = 0; x < 8; ++x) {
+    const int64_t n = (static_cast<int64_t>(a.nlong) * (x + 1) / 8 - static_cast<int64_t>(a.nlong) * x / 8) +
+                      (static_cast<int64_t>(ns) * (x + 1) / 8 - static_cast<int64_t>(ns) * x / 8);
+    most = std::max(most, static_cast<int>((n + kWavesPerBlock - 1) / kWavesPerBlock));
   }
-  return cached;
+  return 8 * most;
+}
+
+template <class P, int SOLVER, bool kMinmod>
+static hipError_t launch_fused(StepArgs<P>& a, int nstrips, hipStream_t st) {
+  thread_local const int wps = waves_per_simd(reinterpret_cast<const void*>(k_step_fused<P, SOLVER, kMinmod>));
+  const int nblocks = plan_segments(a, nstrips, a.g.ny, wps);
+  hipLaunchKernelGGL((k_step_fused<P, SOLVER, kMinmod>), dim3(nblocks), dim3(64 * kWavesPerBlock), 0, st, a);
+  return hipGetLastError();
 }
 
 template <class P>
@@ -283,18 +345,10 @@ static hipError_t fused_step(const e2d_params& p, const DevBuffers& b, const Ste
   a.dt_host = c.dt_host;
   const bool hllc = p.riemann_solver == E2D_RIEMANN_HLLC;
   const bool minmod = !hllc && p.slope_type == 1;
-  a.seg_rows = hllc ? seg_rows_for<P, 2>(p.ny) : (minmod ? seg_rows_for<P, 0, true>(p.ny) : seg_rows_for<P, 0>(p.ny));
-  a.nsegs = (p.ny + a.seg_rows - 1) / a.seg_rows;
-  a.seg_groups = (a.nsegs + kWavesPerBlock - 1) / kWavesPerBlock;
   const int nstrips = (p.nx + kStripOut - 1) / kStripOut;
-  const int nblocks = nstrips * a.seg_groups;
-  if (hllc)
-    hipLaunchKernelGGL((k_step_fused<P, 2>), dim3(nblocks), dim3(64 * kWavesPerBlock), 0, st, a);
-  else if (minmod)
-    hipLaunchKernelGGL((k_step_fused<P, 0, true>), dim3(nblocks), dim3(64 * kWavesPerBlock), 0, st, a);
-  else
-    hipLaunchKernelGGL((k_step_fused<P, 0>), dim3(nblocks), dim3(64 * kWavesPerBlock), 0, st, a);
-  return hipGetLastError();
+  if (hllc) return launch_fused<P, 2, false>(a, nstrips, st);
+  if (minmod) return launch_fused<P, 0, true>(a, nstrips, st);
+  return launch_fused<P, 0, false>(a, nstrips, st);
 }
 
 hipError_t launch_fused_step(const e2d_params& p, const DevBuffers& b, const StepCfg& c, hipStream_t s) {
@@ -357,11 +411,7 @@ hipError_t launch_tangent_step(const e2d_params& p, const DevBuffers& b, int buf
   a.track_dt = 0;
   a.slot_next = 1;
   a.dt_host = dt;
-  a.seg_rows = seg_rows_for<PDual, 0>(p.ny);
-  a.nsegs = (p.ny + a.seg_rows - 1) / a.seg_rows;
-  a.seg_groups = (a.nsegs + kWavesPerBlock - 1) / kWavesPerBlock;
-  const int nblocks = ((p.nx + kStripOut - 1) / kStripOut) * a.seg_groups;
-  hipLaunchKernelGGL((k_step_fused<PDual, 0>), dim3(nblocks), dim3(64 * kWavesPerBlock), 0, st, a);
+  if (hipError_t e = launch_fused<PDual, 0, false>(a, (p.nx + kStripOut - 1) / kStripOut, st)) return e;
   hipLaunchKernelGGL(k_unpack_dual, grid, dim3(256), 0, st, g, static_cast<const Dual*>(dual_out), v_out);
   return hipGetLastError();
 }
@@ -748,7 +798,7 @@ static hipError_t materialised_step(const e2d_params& p, const DevBuffers& b, co
     // stored rows and columns [2, size-2]; segment length as for the fused step
     const bool hllc = p.riemann_solver == E2D_RIEMANN_HLLC;
     const bool minmod = !hllc && p.slope_type == 1;
-    a.seg_rows = hllc ? seg_rows_for<P, 2>(p.ny) : (minmod ? seg_rows_for<P, 0, true>(p.ny) : seg_rows_for<P, 0>(p.ny));
+    a.seg_rows = fused_seg_rows(p.ny);
     a.nsegs = (J_ - 3 + a.seg_rows - 1) / a.seg_rows;
     a.seg_groups = (a.nsegs + kWavesPerBlock - 1) / kWavesPerBlock;
     const dim3 mgrid(((S_ - 3 + kStripOut - 1) / kStripOut) * a.seg_groups);

@@ -154,6 +154,37 @@ def test_sizes_vs_oracle(gpu_lib, nx, ny, bc, impl):
     assert _same(u2, o.get(U2)), _diff_report(u2, o.get(U2))
 
 
+# segment plans of the fused step: long rows, short rows, percent of rows in
+# long segments (E2D_FUSED_SEG_ROWS / _SHORT_ROWS / _LONG_PCT); None = default
+PLANS = [(None, None, None), (5, 2, 60), (3, 1, 0), (7, 3, 100), (1, 1, 75), (40, 7, 90)]
+
+
+@pytest.mark.parametrize("plan", PLANS)
+@pytest.mark.parametrize("prec, kind", [(e2d.PREC_FP64, "f64"), (e2d.PREC_FP32, "f32")])
+def test_segment_plans_vs_oracle(gpu_lib, monkeypatch, plan, prec, kind):
+    """Whatever the split of each strip's rows into long and short segments,
+    and however they are spread over the XCDs, the fused step gives the
+    oracle's bits (device-resident dt loop, 20 steps, 130 x 151)."""
+    for name, v in zip(("E2D_FUSED_SEG_ROWS", "E2D_FUSED_SHORT_ROWS", "E2D_FUSED_LONG_PCT"), plan):
+        if v is None:
+            monkeypatch.delenv(name, raising=False)
+        else:
+            monkeypatch.setenv(name, str(v))
+    nx, ny, steps = 130, 151, 20
+    o = Oracle(nx, ny, precision=kind)
+    o.init(IC_DIAG)
+    _, dts = o.run(steps)
+    o.compute_dt(steps % 2)  # the running maximum that includes the last state, as the device keeps it
+    with e2d.HydroRun(_params(nx, ny, 0, prec)) as run:
+        run.init("diag")
+        run.make_boundaries(0)
+        run.run_steps(steps)
+        u, u2 = run.download(0), run.download(1)
+        assert run.invdt == o.invdt
+    assert _same(u, o.get(U)), _diff_report(u, o.get(U))
+    assert _same(u2, o.get(U2)), _diff_report(u2, o.get(U2))
+
+
 @pytest.mark.parametrize("impl", [0, 1, 2])
 @pytest.mark.parametrize("prec, kind", [(e2d.PREC_FP64, "f64"), (e2d.PREC_FP32, "f32")])
 def test_hllc_runs_vs_oracle(gpu_lib, impl, prec, kind):

@@ -63,33 +63,41 @@ struct StepArgs {
   // [short_base, ny + 2).
   int nsegs, seg_rows, nlong;
   int nshort, short_rows, short_base, nitems;
+  int chunk;  // items per chunk dealt to one XCD (see fused_segment)
 };
+
+// Items of XCD x among n items dealt to the 8 XCDs in chunks of c items,
+// chunk i going to XCD i mod 8.
+__host__ __device__ __forceinline__ int xcd_share(int n, int c, int x) {
+  const int full = n / c, rem = n - full * c;
+  return (full / 8 + (x < full % 8 ? 1 : 0)) * c + (x == full % 8 ? rem : 0);
+}
 
 // Segment (strip, rows [j0, j1)) of wavefront `wave` of block b; false for a
 // wavefront without one.  Blocks go to the 8 XCDs round-robin (b % 8) and
-// start in order on each: XCD x takes the x-th eighth of the long segments,
-// then the x-th eighth of the short ones.  Every XCD thus ends its share on
-// short segments, which shortens the tail where the device runs out of
-// wavefronts, and the segments that meet in one L2 are neighbours.
+// start in order on each.  The long segments are dealt to the XCDs in chunks
+// of `chunk` consecutive items (vertical neighbours, which share rows, meet in
+// one L2; every XCD gets a spread of the domain, so a region of costly
+// segments does not load one XCD), then the short ones the same way: every XCD
+// ends its share on short segments, which shortens the tail where the device
+// runs out of wavefronts.
 template <class P>
 __device__ __forceinline__ bool fused_segment(const StepArgs<P>& a, int b, int wave, int& strip, int& j0,
                                               int& j1) {
   const int x = b % 8;
   const int k = b / 8 * kWavesPerBlock + wave;
-  const int ns = a.nitems - a.nlong;
-  const int l0 = static_cast<int>(static_cast<int64_t>(a.nlong) * x / 8);
-  const int nl = static_cast<int>(static_cast<int64_t>(a.nlong) * (x + 1) / 8) - l0;
-  const int s0 = static_cast<int>(static_cast<int64_t>(ns) * x / 8);
-  const int nsx = static_cast<int>(static_cast<int64_t>(ns) * (x + 1) / 8) - s0;
+  const int c = a.chunk;
+  const int nl = xcd_share(a.nlong, c, x);
   if (k < nl) {
-    const int item = l0 + k;
+    const int item = ((k / c) * 8 + x) * c + k % c;
     strip = item / a.nsegs;
     j0 = 2 + (item - strip * a.nsegs) * a.seg_rows;
     j1 = min(j0 + a.seg_rows, a.short_base);
     return true;
   }
-  if (k < nl + nsx) {
-    const int item = s0 + (k - nl);
+  const int ks = k - nl;
+  if (ks < xcd_share(a.nitems - a.nlong, c, x)) {
+    const int item = ((ks / c) * 8 + x) * c + ks % c;
     strip = item / a.nshort;
     j0 = a.short_base + (item - strip * a.nshort) * a.short_rows;
     j1 = min(j0 + a.short_rows, a.g.ny + 2);
@@ -310,13 +318,12 @@ static int plan_segments(StepArgs<P>& a, int nstrips, int ny, int wps) {
   a.short_rows = r;
   a.nshort = covered < ny ? (ny - covered + r - 1) / r : 0;
   a.nitems = a.nlong + nstrips * a.nshort;
+  a.chunk = std::max(1, env_int("E2D_FUSED_CHUNK", 16));
   // enough blocks for the XCD with the most segments (see fused_segment)
-  const int ns = a.nitems - a.nlong;
   int most = 0;
   for (int x = 0; x < 8; ++x) {
-    const int64_t n = (static_cast<int64_t>(a.nlong) * (x + 1) / 8 - static_cast<int64_t>(a.nlong) * x / 8) +
-                      (static_cast<int64_t>(ns) * (x + 1) / 8 - static_cast<int64_t>(ns) * x / 8);
-    most = std::max(most, static_cast<int>((n + kWavesPerBlock - 1) / kWavesPerBlock));
+    const int n = xcd_share(a.nlong, a.chunk, x) + xcd_share(a.nitems - a.nlong, a.chunk, x);
+    most = std::max(most, (n + kWavesPerBlock - 1) / kWavesPerBlock);
   }
   return 8 * most;
 }

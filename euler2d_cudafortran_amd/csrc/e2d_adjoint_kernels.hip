@@ -52,7 +52,24 @@ struct AdjArgs {
   double dt;
   int nsegs, seg_rows, seg_groups;     // k_adj_march decomposition
   int gnsegs, gseg_rows, gseg_groups;  // k_adj_gather_march decomposition
+  // k_adj_march_gather / k_adj_seam: per strip, nsegs long segments of
+  // seg_rows completed rows from row 1, then nshort short ones of short_rows
+  // rows from short_base to jsize - 1; dealt to the XCDs as dealt_item does
+  // (nlong = strips * nsegs long items, nitems in all, chunks of `chunk`)
+  int nshort, short_rows, short_base, nlong, nitems, chunk;
 };
+
+// completed rows [ra, rb) of segment sg of a strip (long ones first)
+template <class P>
+__device__ __forceinline__ void adj_segment_rows(const AdjArgs<P>& a, int sg, int& ra, int& rb) {
+  if (sg < a.nsegs) {
+    ra = 1 + sg * a.seg_rows;
+    rb = min(ra + a.seg_rows, a.short_base);
+  } else {
+    ra = a.short_base + (sg - a.nsegs) * a.short_rows;
+    rb = min(ra + a.short_rows, a.g.jsize - 1);
+  }
+}
 
 template <class X, class T>
 __device__ __forceinline__ void load4(const X* a, const Grid& g, int i, int j, T x[4]) {
@@ -545,13 +562,20 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
   using T = typename P::T;
   using U = typename P::U;
   using M = typename P::M;
-  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  int strip, group;
-  strip_and_group<E2D_BAND_MARCH>(lb, a.seg_groups, strip, group);
-  const int seg = group * kAdjWaves + wave;
-  if (seg >= a.nsegs) return;  // whole wavefront leaves: no barrier in this kernel
+  int item;
+  bool is_long;
+  if (!dealt_item(blockIdx.x, wave, kAdjWaves, a.nlong, a.nitems, a.chunk, item, is_long))
+    return;  // whole wavefront leaves: no barrier in this kernel
+  int strip, sg;
+  if (is_long) {
+    strip = item / a.nsegs;
+    sg = item - strip * a.nsegs;
+  } else {
+    strip = item / a.nshort;
+    sg = a.nsegs + (item - strip * a.nshort);
+  }
 
   const Grid& g = a.g;
   const int ci = strip * kFusedStripOut + lane - 3;  // column of this lane
@@ -561,8 +585,8 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
   const bool xface = lane >= 2 && lane <= 62 && ci >= 2 && ci <= g.nx + 2;  // face ci-1/2
   const bool yface = lane >= 2 && lane < 62 && ci >= 2 && ci <= g.nx + 1;
   const bool gat_col = lane >= 3 && lane <= 60 && ci >= 0 && ci < g.isize;  // gathered on this lane
-  const int ra = 1 + seg * a.seg_rows;
-  const int rb = min(ra + a.seg_rows, g.jsize - 1);  // completed rows [ra, rb)
+  int ra, rb;  // completed rows [ra, rb)
+  adj_segment_rows(a, sg, ra, rb);
   const bool limited = slopes_limited(a.h);
   const T st = static_cast<T>(a.h.slope_type);
   const T skip = -T(0);
@@ -761,7 +785,7 @@ __global__ __launch_bounds__(256) void k_adj_seam(const AdjArgs<P> a) {
   const Grid& g = a.g;
   const int ci = blockIdx.x * 64 + (threadIdx.x & 63);
   const int t = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (ci >= g.isize || t >= 2 + 2 * a.nsegs) return;
+  if (ci >= g.isize || t >= 2 + 2 * (a.nsegs + a.nshort)) return;
   const T skip = -T(0);
   auto V = [&](int row, int slot, int v) { return a.seam[seam_at(g, row, slot, v, ci)]; };
   const int last = g.jsize - 2;  // last completed row
@@ -776,9 +800,8 @@ __global__ __launch_bounds__(256) void k_adj_seam(const AdjArgs<P> a) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) qb[v] = T(0) + V(last, 3, v);
   } else {
-    const int sg = (t - 2) >> 1;
-    const int ra = 1 + sg * a.seg_rows;
-    const int rb = min(ra + a.seg_rows, g.jsize - 1);
+    int ra, rb;
+    adj_segment_rows(a, (t - 2) >> 1, ra, rb);
     if (((t - 2) & 1) == 0) {
       r = ra;
 #pragma unroll
@@ -940,6 +963,45 @@ int march_seg_rows(int rows) {
   return r < 4 ? 4 : (r > 16 ? 16 : r);
 }
 
+static int adj_env(const char* name, int dflt) {
+  const char* v = getenv(name);
+  return v && *v ? atoi(v) : dflt;
+}
+
+// Segment plan of k_adj_march_gather over nstrips strips of `rows` completed
+// rows: long segments of about rows/170 rows (within [4, 16];
+// E2D_ADJOINT_SEG_ROWS overrides) over the lower 75% of the rows
+// (E2D_ADJOINT_LONG_PCT), short ones of a quarter of that
+// (E2D_ADJOINT_SHORT_ROWS) over the rest, dealt to the XCDs in chunks of 16
+// segments (E2D_ADJOINT_CHUNK), so every XCD ends on short segments.
+// Measured at 2048^2 with a dense lam, us per adjoint step against uniform
+// 6-row segments: FP64 765 -> 721, FP32 481 -> 437, mixed 417 -> 384; 16-row
+// long segments suit FP64 (715) but not the float kernels, which hold 3
+// wavefronts per SIMD (518 and 440).  Returns the number of blocks.
+template <class P>
+int plan_adjoint_segments(AdjArgs<P>& a, int nstrips, int rows) {
+  int R = adj_env("E2D_ADJOINT_SEG_ROWS", 0);
+  if (R < 1) {
+    R = static_cast<int>(rows / 170.0 + 0.5);
+    R = R < 4 ? 4 : (R > 16 ? 16 : R);
+  }
+  const int r = adj_env("E2D_ADJOINT_SHORT_ROWS", R / 4);
+  const int pct = adj_env("E2D_ADJOINT_LONG_PCT", 75);
+  const int L = pct >= 100 ? (rows + R - 1) / R
+                           : static_cast<int>(static_cast<int64_t>(rows) * (pct < 0 ? 0 : pct) / 100 / R);
+  const int covered = L * R < rows ? L * R : rows;
+  a.seg_rows = R;
+  a.nsegs = L;
+  a.short_rows = r < 1 ? 1 : r;
+  a.short_base = 1 + covered;
+  a.nshort = covered < rows ? (rows - covered + a.short_rows - 1) / a.short_rows : 0;
+  a.nlong = nstrips * L;
+  a.nitems = a.nlong + nstrips * a.nshort;
+  const int c = adj_env("E2D_ADJOINT_CHUNK", 16);
+  a.chunk = c < 1 ? 1 : c;
+  return dealt_blocks(a.nlong, a.nitems, a.chunk, kAdjWaves);
+}
+
 // Rows per k_adj_gather_march segment (E2D_ADJOINT_GATHER_ROWS overrides):
 // short, so that the light, memory-bound kernel has many wavefronts in flight.
 int gather_seg_rows() {
@@ -983,12 +1045,11 @@ hipError_t adjoint_step(const e2d_params& p, const DevBuffers& b, int buf, doubl
                        a);
     hipLaunchKernelGGL(k_adj_gather<P>, dim3((g.isize + 63) / 64, (g.jsize + 3) / 4), dim3(256), 0, st, a);
   } else if (passes == Passes::kFused) {
-    a.seg_rows = march_seg_rows(g.jsize - 2);
-    a.nsegs = (g.jsize - 2 + a.seg_rows - 1) / a.seg_rows;
-    a.seg_groups = (a.nsegs + kAdjWaves - 1) / kAdjWaves;
     const int nstrips = (g.isize + kFusedStripOut - 1) / kFusedStripOut;
-    hipLaunchKernelGGL(k_adj_march_gather<P>, dim3(nstrips * a.seg_groups), dim3(64 * kAdjWaves), 0, st, a);
-    hipLaunchKernelGGL(k_adj_seam<P>, dim3((g.isize + 63) / 64, (2 + 2 * a.nsegs + 3) / 4), dim3(256), 0, st, a);
+    const int nblocks = plan_adjoint_segments(a, nstrips, g.jsize - 2);
+    hipLaunchKernelGGL(k_adj_march_gather<P>, dim3(nblocks), dim3(64 * kAdjWaves), 0, st, a);
+    const int per_strip = a.nsegs + a.nshort;
+    hipLaunchKernelGGL(k_adj_seam<P>, dim3((g.isize + 63) / 64, (2 + 2 * per_strip + 3) / 4), dim3(256), 0, st, a);
   } else {
     a.seg_rows = march_seg_rows(g.jsize - 2);
     a.nsegs = (g.jsize - 2 + a.seg_rows - 1) / a.seg_rows;

@@ -224,6 +224,49 @@ __device__ __forceinline__ void invdt_fold(double* scal, int slot, int who, doub
   atomic_max_nonneg(scal + slot * kInvdtSlot + (who % kInvdtParts) * kInvdtStride, v);
 }
 
+// Segments dealt to the XCDs.  Workgroups reach the 8 XCDs round-robin
+// (block b to XCD b mod 8) and start in order on each.  nitems items, the
+// first nlong of them long segments, are dealt in chunks of c consecutive
+// items, chunk i to XCD i mod 8: the long items first, then the short ones
+// (chunks counted afresh), so that every XCD ends its share on short items.
+// Items of XCD x among n items dealt that way:
+__host__ __device__ __forceinline__ int xcd_share(int n, int c, int x) {
+  const int full = n / c, rem = n - full * c;
+  return (full / 8 + (x < full % 8 ? 1 : 0)) * c + (x == full % 8 ? rem : 0);
+}
+
+// The item of wavefront `wave` of block b (wpb wavefronts per block): its
+// index among the long items (is_long) or among the short ones; false when
+// the wavefront has none.
+__device__ __forceinline__ bool dealt_item(int b, int wave, int wpb, int nlong, int nitems, int c, int& item,
+                                           bool& is_long) {
+  const int x = b % 8;
+  const int k = b / 8 * wpb + wave;
+  const int nl = xcd_share(nlong, c, x);
+  if (k < nl) {
+    item = ((k / c) * 8 + x) * c + k % c;
+    is_long = true;
+    return true;
+  }
+  const int ks = k - nl;
+  if (ks < xcd_share(nitems - nlong, c, x)) {
+    item = ((ks / c) * 8 + x) * c + ks % c;
+    is_long = false;
+    return true;
+  }
+  return false;
+}
+
+// blocks that give every XCD its share
+inline int dealt_blocks(int nlong, int nitems, int c, int wpb) {
+  int most = 0;
+  for (int x = 0; x < 8; ++x) {
+    const int n = xcd_share(nlong, c, x) + xcd_share(nitems - nlong, c, x);
+    most = most > (n + wpb - 1) / wpb ? most : (n + wpb - 1) / wpb;
+  }
+  return 8 * most;
+}
+
 // XCD-aware block order: hardware hands consecutive workgroups to the 8 XCDs
 // in turn; this gives each XCD a contiguous range of logical blocks so that
 // neighbouring strips/segments (which share halo rows and columns) meet in the

@@ -66,13 +66,6 @@ struct StepArgs {
   int chunk;  // items per chunk dealt to one XCD (see fused_segment)
 };
 
-// Items of XCD x among n items dealt to the 8 XCDs in chunks of c items,
-// chunk i going to XCD i mod 8.
-__host__ __device__ __forceinline__ int xcd_share(int n, int c, int x) {
-  const int full = n / c, rem = n - full * c;
-  return (full / 8 + (x < full % 8 ? 1 : 0)) * c + (x == full % 8 ? rem : 0);
-}
-
 // Segment (strip, rows [j0, j1)) of wavefront `wave` of block b; false for a
 // wavefront without one.  Blocks go to the 8 XCDs round-robin (b % 8) and
 // start in order on each.  The long segments are dealt to the XCDs in chunks
@@ -84,26 +77,19 @@ __host__ __device__ __forceinline__ int xcd_share(int n, int c, int x) {
 template <class P>
 __device__ __forceinline__ bool fused_segment(const StepArgs<P>& a, int b, int wave, int& strip, int& j0,
                                               int& j1) {
-  const int x = b % 8;
-  const int k = b / 8 * kWavesPerBlock + wave;
-  const int c = a.chunk;
-  const int nl = xcd_share(a.nlong, c, x);
-  if (k < nl) {
-    const int item = ((k / c) * 8 + x) * c + k % c;
+  int item;
+  bool is_long;
+  if (!dealt_item(b, wave, kWavesPerBlock, a.nlong, a.nitems, a.chunk, item, is_long)) return false;
+  if (is_long) {
     strip = item / a.nsegs;
     j0 = 2 + (item - strip * a.nsegs) * a.seg_rows;
     j1 = min(j0 + a.seg_rows, a.short_base);
-    return true;
-  }
-  const int ks = k - nl;
-  if (ks < xcd_share(a.nitems - a.nlong, c, x)) {
-    const int item = ((ks / c) * 8 + x) * c + ks % c;
+  } else {
     strip = item / a.nshort;
     j0 = a.short_base + (item - strip * a.nshort) * a.short_rows;
     j1 = min(j0 + a.short_rows, a.g.ny + 2);
-    return true;
   }
-  return false;
+  return true;
 }
 
 // Load cell (ci, r) of the input, boundary-filled (storage type); write the
@@ -319,13 +305,7 @@ static int plan_segments(StepArgs<P>& a, int nstrips, int ny, int wps) {
   a.nshort = covered < ny ? (ny - covered + r - 1) / r : 0;
   a.nitems = a.nlong + nstrips * a.nshort;
   a.chunk = std::max(1, env_int("E2D_FUSED_CHUNK", 16));
-  // enough blocks for the XCD with the most segments (see fused_segment)
-  int most = 0;
-  for (int x = 0; x < 8; ++x) {
-    const int n = xcd_share(a.nlong, a.chunk, x) + xcd_share(a.nitems - a.nlong, a.chunk, x);
-    most = std::max(most, (n + kWavesPerBlock - 1) / kWavesPerBlock);
-  }
-  return 8 * most;
+  return dealt_blocks(a.nlong, a.nitems, a.chunk, kWavesPerBlock);
 }
 
 template <class P, int SOLVER, bool kMinmod>

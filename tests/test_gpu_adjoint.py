@@ -450,22 +450,28 @@ def test_marching_adjoint_equals_separate_passes(gpu_lib, prec, monkeypatch):
     of q and of the slopes through HBM) and the gather passes
     (E2D_ADJOINT_PASSES=separate, face bars through HBM as well), on odd sizes,
     thin grids, every boundary type, and row segments of 1, 2, 3 and 5 rows
-    (the last segment of a grid may be shorter than the others); and each in
-    place (lam_out = lam_in, which runs march where fused was chosen)."""
+    (the last segment of a grid may be shorter than the others), and plans
+    that mix long and short segments (rows, short rows, percent of rows in
+    long segments; fused only); and each in place (lam_out = lam_in, which
+    runs march where fused was chosen)."""
     cases = [((1, 1, 1, 1), (96, 80), None), ((3, 3, 3, 3), (61, 37), None), ((1, 3, 2, 1), (130, 7), None),
              ((2, 2, 2, 2), (4, 4), None), ((1, 1, 3, 3), (64, 200), None), ((1, 1, 1, 1), (96, 80), "3"),
-             ((3, 1, 2, 3), (70, 21), "2"), ((1, 2, 3, 1), (33, 45), "1"), ((2, 3, 1, 1), (50, 60), "5")]
+             ((3, 1, 2, 3), (70, 21), "2"), ((1, 2, 3, 1), (33, 45), "1"), ((2, 3, 1, 1), (50, 60), "5"),
+             ((1, 1, 1, 1), (96, 80), ("5", "2", "60")), ((3, 1, 2, 3), (70, 21), ("4", "1", "50")),
+             ((1, 1, 3, 3), (64, 200), ("7", "3", "75")), ((2, 2, 2, 2), (40, 9), ("3", "2", "0"))]
     for bc, (nx, ny), seg_rows in cases:
+        plan = seg_rows if isinstance(seg_rows, tuple) else (seg_rows, None, None)
+        for name, v in zip(("E2D_ADJOINT_SEG_ROWS", "E2D_ADJOINT_SHORT_ROWS", "E2D_ADJOINT_LONG_PCT"), plan):
+            if v is None:
+                monkeypatch.delenv(name, raising=False)
+            else:
+                monkeypatch.setenv(name, v)
         p = _params(nx, ny, precision=prec, boundary_type_xmin=bc[0], boundary_type_xmax=bc[1],
                     boundary_type_ymin=bc[2], boundary_type_ymax=bc[3])
         lam = np.random.default_rng(nx * ny).normal(size=(nx + 4, ny + 4, 4))
         res = {}
         for mode in ("separate", "march", "fused"):
             monkeypatch.setenv("E2D_ADJOINT_PASSES", mode)
-            if seg_rows is None:
-                monkeypatch.delenv("E2D_ADJOINT_SEG_ROWS", raising=False)
-            else:
-                monkeypatch.setenv("E2D_ADJOINT_SEG_ROWS", seg_rows)
             with e2d.HydroRun(p) as run:
                 U = _evolved_diag(run, 25)
                 run.upload(0, U)

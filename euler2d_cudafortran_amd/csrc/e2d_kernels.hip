@@ -548,10 +548,13 @@ struct MatArgs {
   int nsegs, seg_rows, seg_groups;  // k_flux_store row segments
 };
 
-template <class P>
+// kWave: called with all 64 lanes of the wavefront active (wave-parallel read)
+template <bool kWave = false, class P>
 __device__ __forceinline__ void step_dt(const MatArgs<P>& a, typename P::U& dtdx, typename P::U& dtdy) {
   using U = typename P::U;
-  const U dt = a.dt_from_device ? a.hU.cfl / static_cast<U>(invdt_read(a.scal, a.slot_cur)) : static_cast<U>(a.dt_host);
+  U dt = static_cast<U>(a.dt_host);
+  if (a.dt_from_device)
+    dt = a.hU.cfl / static_cast<U>(kWave ? invdt_read_wave(a.scal, a.slot_cur) : invdt_read(a.scal, a.slot_cur));
   dtdx = dt / a.hU.dx;
   dtdy = dt / a.hU.dy;
 }
@@ -687,7 +690,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_flux_store(const MatArg
   const int j0 = 2 + seg * a.seg_rows;
   const int j1 = min(j0 + a.seg_rows, g.jsize - 1);  // stored rows [j0, j1)
   U dtdxU, dtdyU;
-  step_dt(a, dtdxU, dtdyU);
+  step_dt<true>(a, dtdxU, dtdyU);
   const T dtdx = static_cast<T>(dtdxU), dtdy = static_cast<T>(dtdyU);
   const Hydro<T>& h = a.hT;
 
@@ -785,7 +788,16 @@ static hipError_t materialised_step(const e2d_params& p, const DevBuffers& b, co
     // stored rows and columns [2, size-2]; segment length as for the fused step
     const bool hllc = p.riemann_solver == E2D_RIEMANN_HLLC;
     const bool minmod = !hllc && p.slope_type == 1;
-    a.seg_rows = fused_seg_rows(p.ny);
+    const void* kernel = hllc ? reinterpret_cast<const void*>(k_flux_store<P, 2>)
+                              : (minmod ? reinterpret_cast<const void*>(k_flux_store<P, 0, true>)
+                                        : reinterpret_cast<const void*>(k_flux_store<P, 0>));
+    thread_local const void* cached_kernel = nullptr;
+    thread_local int cached_wps = 3;
+    if (kernel != cached_kernel) {
+      cached_wps = waves_per_simd(kernel);
+      cached_kernel = kernel;
+    }
+    a.seg_rows = fused_seg_rows(p.ny, cached_wps);
     a.nsegs = (J_ - 3 + a.seg_rows - 1) / a.seg_rows;
     a.seg_groups = (a.nsegs + kWavesPerBlock - 1) / kWavesPerBlock;
     const dim3 mgrid(((S_ - 3 + kStripOut - 1) / kStripOut) * a.seg_groups);

@@ -276,6 +276,18 @@ static int fused_seg_rows(int ny, int waves_per_simd = 3) {
   return d < 1 ? 1 : (d > 32 ? 32 : d);
 }
 
+// compute units of the current device (256 on MI355X), per host thread
+static int device_cus() {
+  thread_local int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+  }
+  return cus;
+}
+
 // resident wavefronts per SIMD of a kernel launched with kWavesPerBlock waves per block
 static int waves_per_simd(const void* kernel) {
   int blocks_per_cu = 0;
@@ -286,16 +298,42 @@ static int waves_per_simd(const void* kernel) {
   return std::max(1, blocks_per_cu * kWavesPerBlock / 4);
 }
 
-// Segment plan of a fused launch over nstrips strips: long segments over the
-// lower 75% of each strip's rows (E2D_FUSED_LONG_PCT), short ones of a quarter
-// of their length over the rest (E2D_FUSED_SHORT_ROWS); the environment is
-// read at every launch.  Returns the number of blocks.
+// Segment plan of a fused launch over nstrips strips.  Long segments cover
+// about 80% of each strip's rows, in as many segments per strip as fill k whole
+// rounds of the device's resident wavefronts (k the least that keeps them
+// within 40 rows); short ones of a quarter of their length cover the rest.
+// Every long segment then starts in the first k rounds, and what is left when
+// they end is short work, so no long segment runs alone at the end of the
+// launch.  Measured on MI355X (us per step, dt on the device): 2048^2 FP64
+// 19-row long segments (one round) 213.5 against 223 with 16 rows over 75%
+// (which leaves 9% of the long segments for a second round); FP32 12 rows
+// 134-136 against 139-140; mixed 106.3 against 107.1; 4096^2 FP64 37 rows
+// (two rounds) 711 against 716, 74 rows (one round) 719.  Where the rounds
+// would give segments of fewer than 8 rows (grids up to about 1024^2) the
+// plan is long segments of fused_seg_rows rows over 75% of the rows.
+// E2D_FUSED_SEG_ROWS or E2D_FUSED_LONG_PCT select that plan with the given
+// values, E2D_FUSED_SHORT_ROWS the short length; the environment is read at
+// every launch.  Returns the number of blocks.
 template <class P>
-static int plan_segments(StepArgs<P>& a, int nstrips, int ny, int wps) {
-  const int R = fused_seg_rows(ny, wps);
+static int plan_segments(StepArgs<P>& a, int nstrips, int ny, int wps, int resident) {
+  int R = 0, L = 0;
+  const bool forced = env_int("E2D_FUSED_SEG_ROWS", 0) >= 1 || env_int("E2D_FUSED_LONG_PCT", -1) >= 0;
+  const int round = resident / std::max(1, nstrips);  // long segments per strip in one round
+  if (!forced && round >= 1) {
+    const double rows80 = 0.8 * ny;
+    int k = 1;
+    while (rows80 / (k * round) > 40.0) ++k;
+    R = static_cast<int>(rows80 / (k * round) + 0.5);
+    L = k * round;
+    if (R < 8) R = 0;  // small grids: the percentage plan below
+    while (R > 0 && L * R > ny) --R;
+  }
+  if (R == 0) {
+    R = fused_seg_rows(ny, wps);
+    const int pct = std::min(100, std::max(0, env_int("E2D_FUSED_LONG_PCT", 75)));
+    L = pct >= 100 ? (ny + R - 1) / R : static_cast<int>(static_cast<int64_t>(ny) * pct / 100 / R);
+  }
   const int r = std::max(1, env_int("E2D_FUSED_SHORT_ROWS", R / 4));
-  const int pct = std::min(100, std::max(0, env_int("E2D_FUSED_LONG_PCT", 75)));
-  const int L = pct >= 100 ? (ny + R - 1) / R : static_cast<int>(static_cast<int64_t>(ny) * pct / 100 / R);
   const int covered = std::min(L * R, ny);
   a.seg_rows = R;
   a.nsegs = L;
@@ -311,7 +349,7 @@ static int plan_segments(StepArgs<P>& a, int nstrips, int ny, int wps) {
 template <class P, int SOLVER, bool kMinmod>
 static hipError_t launch_fused(StepArgs<P>& a, int nstrips, hipStream_t st) {
   thread_local const int wps = waves_per_simd(reinterpret_cast<const void*>(k_step_fused<P, SOLVER, kMinmod>));
-  const int nblocks = plan_segments(a, nstrips, a.g.ny, wps);
+  const int nblocks = plan_segments(a, nstrips, a.g.ny, wps, 4 * wps * device_cus());
   hipLaunchKernelGGL((k_step_fused<P, SOLVER, kMinmod>), dim3(nblocks), dim3(64 * kWavesPerBlock), 0, st, a);
   return hipGetLastError();
 }

@@ -259,16 +259,17 @@ static int env_int(const char* name, int dflt) {
   return v && *v ? atoi(v) : dflt;
 }
 
-// Rows per long segment.  Each segment re-primes its rolling rows (two extra
-// row loads and primitive conversions, one extra row of y-faces), so long
-// segments are cheaper per row; short ones give more wavefronts to hide
+// Rows per long segment where the whole-round plan does not apply (small
+// grids; the v2 flux kernel).  Each segment re-primes its rolling rows (two
+// extra row loads and primitive conversions, one extra row of y-faces), so
+// long segments are cheaper per row; short ones give more wavefronts to hide
 // latency and balance the data-dependent Newton work, and on small grids a
 // wavefront's rows are the critical path.  About ny/128 rows at 3 resident
-// wavefronts per SIMD (the FP64 kernels), proportionally fewer where more
-// are resident, within [1, 32]; E2D_FUSED_SEG_ROWS overrides it.  Measured on
-// MI355X with the 75/25 split below: 16 rows at 2048^2 (FP64 228-234 us/step
-// against 251 us for uniform ny/170-row segments on the same device), 32 at
-// 4096^2, 8 at 1024^2.
+// wavefronts per SIMD (the FP64 kernels), proportionally fewer where more are
+// resident, within [1, 32]; E2D_FUSED_SEG_ROWS overrides it.  Measured on
+// MI355X with long segments over 75% of the rows: 8 rows at 1024^2 (76 us per
+// FP64 step against 86 us for uniform ny/170-row segments), 4 at 512^2 (29.6
+// against 35.6).
 static int fused_seg_rows(int ny, int waves_per_simd = 3) {
   const int r = env_int("E2D_FUSED_SEG_ROWS", 0);
   if (r >= 1) return r;

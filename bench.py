@@ -133,8 +133,12 @@ def variant(e2d, nx, impl, prec, steps, warmup):
         wall, kms = time_steps(run, steps, warmup)
     finally:
         run.close()
+    # algorithmic HBM bytes per step: the 4 conserved variables read and
+    # written once per cell, in the storage type (8 B in FP64, 4 B in FP32 and mixed)
+    alg = nx * nx * 4 * 2 * (8 if prec == e2d.PREC_FP64 else 4)
     return {"mcell_per_s": round(nx * nx * steps / wall / 1e6, 1), "steps": steps,
-            "ms_per_step": round(1e3 * wall / steps, 4), "kernel_ms": round(kms, 4)}
+            "ms_per_step": round(1e3 * wall / steps, 4), "kernel_ms": round(kms, 4),
+            "hbm_frac_algorithmic": round(alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
 def host_loop(e2d, nx, steps, warmup=10):

@@ -210,13 +210,20 @@ __device__ __forceinline__ double invdt_read(const double* scal, int slot) {
 // The same for a whole wavefront (all 64 lanes active): lane l loads part
 // l mod kInvdtParts, one round trip for the slot, then a wave maximum; the
 // result is moved to scalar registers, where the uniform dt arithmetic stays.
-__device__ __forceinline__ double invdt_read_wave(const double* scal, int slot) {
+// In two halves, so that a kernel can issue the load early and reduce later:
+__device__ __forceinline__ double invdt_part(const double* scal, int slot) {
   const int lane = static_cast<int>(threadIdx.x & 63);
-  const double m = wave_max(scal[slot * kInvdtSlot + (lane % kInvdtParts) * kInvdtStride]);
+  return scal[slot * kInvdtSlot + (lane % kInvdtParts) * kInvdtStride];
+}
+__device__ __forceinline__ double invdt_reduce(double part) {
+  const double m = wave_max(part);
   const long long bits = __double_as_longlong(m);
   const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(bits)));
   const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(bits >> 32)));
   return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+__device__ __forceinline__ double invdt_read_wave(const double* scal, int slot) {
+  return invdt_reduce(invdt_part(scal, slot));
 }
 
 // fold a non-negative rate into slot `slot`, part chosen by `who`

@@ -92,18 +92,26 @@ __device__ __forceinline__ bool fused_segment(const StepArgs<P>& a, int b, int w
   return true;
 }
 
-// Load cell (ci, r) of the input, boundary-filled (storage type); write the
-// ghost value back so the input buffer ends up as make_boundaries leaves it.
+// Loads of cell (ci, r) of the input, taken from its boundary source as
+// stored (ghost cells are never read: every ghost maps to an interior cell,
+// corners to themselves), so the loads of several rows can be in flight at
+// once.  source_cell applies the reflection's sign flip when the values are
+// used and writes ghost values back, so that the input buffer ends up as
+// make_boundaries leaves it; doing that here, right after the loads, would
+// make every load wait for its data.
 template <class P>
-__device__ __forceinline__ void load_cell(const StepArgs<P>& a, int ci, bool col_ok, int r,
-                                          typename P::S raw[4]) {
-  using S = typename P::S;
+__device__ __forceinline__ void fetch_cell(const StepArgs<P>& a, int ci, int r, typename P::S raw[4]) {
   const Src s = ghost_source(a.g, ci, r);
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const S x = a.uin[at(a.g, s.i, s.j, v)];
-    raw[v] = (v == s.flip) ? -x : x;
-  }
+  for (int v = 0; v < 4; ++v) raw[v] = a.uin[at(a.g, s.i, s.j, v)];
+}
+
+template <class P>
+__device__ __forceinline__ void source_cell(const StepArgs<P>& a, int ci, bool col_ok, int r,
+                                            typename P::S raw[4]) {
+  const Src s = ghost_source(a.g, ci, r);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) raw[v] = (v == s.flip) ? -raw[v] : raw[v];
   if (s.ghost && col_ok) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) a.uin[at(a.g, ci, r, v)] = raw[v];
@@ -139,11 +147,27 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) E2D_FUSED_ATTR void k_step_fus
   const int cl = col_ok ? ci : g.isize - 1;  // clamped column for loads
   const bool out_lane = (lane >= 2) && (lane < 62) && (ci <= g.nx + 1);
 
+  const Hydro<T>& h = a.hT;
+  U rate_max = U(0);
+
+  // the running invDt and the first three rows are all requested before any
+  // of them is used, so their round trips overlap
+  double invdt_cur = 0.0;
+  if (a.dt_from_device) invdt_cur = invdt_part(a.scal, a.slot_cur);
+  T qa[4], qb[4], qc[4];
+  S raw[4], raw_next[4], raw_b[4];
+  fetch_cell<P>(a, cl, j0 - 2, raw);
+  fetch_cell<P>(a, cl, j0 - 1, raw_b);
+  fetch_cell<P>(a, cl, j0, raw_next);
+  source_cell<P>(a, cl, col_ok, j0 - 2, raw);
+  prim_of<P>(h, raw, qa);
+  source_cell<P>(a, cl, col_ok, j0 - 1, raw_b);
+  prim_of<P>(h, raw_b, qb);
+
   // dt of this step (serial_ref/m_run.f90:308, 384-385)
   U dt;
-  double invdt_cur = 0.0;
   if (a.dt_from_device) {
-    invdt_cur = invdt_read_wave(a.scal, a.slot_cur);
+    invdt_cur = invdt_reduce(invdt_cur);
     dt = a.hU.cfl / static_cast<U>(invdt_cur);
   } else {
     dt = static_cast<U>(a.dt_host);
@@ -152,16 +176,6 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) E2D_FUSED_ATTR void k_step_fus
   const U dtdyU = dt / a.hU.dy;
   const T dtdx = static_cast<T>(dtdxU);
   const T dtdy = static_cast<T>(dtdyU);
-  const Hydro<T>& h = a.hT;
-  U rate_max = U(0);
-
-  T qa[4], qb[4], qc[4];
-  S raw[4], raw_next[4];
-  load_cell<P>(a, cl, col_ok, j0 - 2, raw);
-  prim_of<P>(h, raw, qa);
-  load_cell<P>(a, cl, col_ok, j0 - 1, raw);
-  prim_of<P>(h, raw, qb);
-  load_cell<P>(a, cl, col_ok, j0, raw_next);
 
   T ymax_prev[4];
   U fy_prev[4] = {U(0), U(0), U(0), U(0)}, fx_prev[4] = {U(0), U(0), U(0), U(0)};
@@ -173,13 +187,14 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) E2D_FUSED_ATTR void k_step_fus
     // in flight during the Riemann solves
 #pragma unroll
     for (int v = 0; v < 4; ++v) raw[v] = raw_next[v];
-    if (r + 2 <= j1 + 1) load_cell<P>(a, cl, col_ok, r + 2, raw_next);
+    if (r + 2 <= j1 + 1) fetch_cell<P>(a, cl, r + 2, raw_next);
     S uold[4];
     const bool do_update = (r >= j0 + 1);
     if (do_update && out_lane) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) uold[v] = a.uin[at(g, ci, r - 1, v)];
     }
+    source_cell<P>(a, cl, col_ok, r + 1, raw);
     prim_of<P>(h, raw, qc);
 
     // slopes and trace sources of row r

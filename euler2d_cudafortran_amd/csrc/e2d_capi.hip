@@ -237,12 +237,34 @@ void free_all(e2d_ctx* c) {
   if (c->stream) (void)hipStreamDestroy(c->stream), c->stream = nullptr;
 }
 
-// Binomial checkpointing (Griewank's treeverse).  beta(s, t) = C(s+t, s) steps
-// can be reversed with s snapshots and at most t forward sweeps per step.
-double beta(int s, int t) {
-  double r = 1.0;
-  for (int k = 1; k <= s; ++k) r = r * (t + k) / k;
-  return r;
+// Binomial checkpointing (Griewank and Walther, Algorithm 799: revolve, ACM
+// TOMS 26 (2000)).
+// beta(c, r) = C(c+r, c): the most steps that c kept states (the first one
+// included) reverse when no step is advanced more than r times.
+double beta(int c, int r) {
+  if (r < 0) return 0.0;
+  double v = 1.0;
+  for (int k = 1; k <= c; ++k) v = v * (r + k) / k;
+  return v;
+}
+
+// Where to store the next snapshot when reversing l steps whose first state is
+// kept, with s snapshot slots free: m steps on.  With c = s + 1 states and r
+// the least repetition count with beta(c, r) >= l, every m with
+//   max(l - beta(c-1, r), beta(c, r-2) + 1) <= m <= min(beta(c, r-1), l - beta(c-1, r-1) - 1)
+// leaves the right part l - m to c - 1 states at r repetitions and the left
+// part m to c states at r - 1, so the whole reversal advances the least
+// number of steps, r l - beta(c+1, r-1); this takes the largest such m.
+int split_point(int l, int s) {
+  const int c = s + 1;
+  int r = 1;
+  while (beta(c, r) < l) ++r;
+  const double lo = std::max(l - beta(c - 1, r), beta(c, r - 2) + 1.0);
+  const double hi = std::min(beta(c, r - 1), l - beta(c - 1, r - 1) - 1.0);
+  double m = hi >= lo ? hi : lo;
+  if (m > l - 1) m = l - 1;
+  if (m < 1) m = 1;
+  return static_cast<int>(m);
 }
 
 // (step, slot) of the snapshots that Rollout::reverse stores on its first
@@ -257,11 +279,7 @@ std::vector<std::pair<int, int>> first_descent(int nsteps, int nsnap) {
     const int l = b - a;
     const int s = static_cast<int>(free_slots.size());
     if (l == 1 || s == 0) break;
-    int t = 1;
-    while (beta(s, t) < l) ++t;
-    int m = static_cast<int>(l - beta(s - 1, t));
-    if (m < 1) m = 1;
-    if (m > l - 1) m = l - 1;
+    const int m = split_point(l, s);
     const int mine = free_slots.back();
     free_slots.pop_back();
     out.emplace_back(a + m, mine);
@@ -664,11 +682,7 @@ struct Rollout {
       }
       return E2D_OK;
     }
-    int t = 1;
-    while (beta(s, t) < l) ++t;
-    int m = static_cast<int>(l - beta(s - 1, t));
-    if (m < 1) m = 1;
-    if (m > l - 1) m = l - 1;
+    const int m = split_point(l, s);
     const int mine = free_slots.back();
     free_slots.pop_back();
     if (!pre.empty() && pre.front() == std::make_pair(a + m, mine)) {

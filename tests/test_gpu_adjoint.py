@@ -131,19 +131,40 @@ def _record_forward(run, U0, nsteps):
     return np.array(dts)
 
 
-def _first_descent_steps(nsteps, nsnap):
-    """Forward steps of the reverse sweep's first descent (treeverse with
-    Griewank's split), i.e. what a recorded forward saves the sweep."""
+def _beta(c, r):
     from math import comb
+    return comb(c + r, c) if r >= 0 else 0
+
+
+def _split(l, s):
+    """Griewank and Walther's binomial split: the largest m in the range that
+    gives the least number of forward steps (c = s + 1 kept states)."""
+    c, r = s + 1, 1
+    while _beta(c, r) < l:
+        r += 1
+    lo = max(l - _beta(c - 1, r), _beta(c, r - 2) + 1)
+    hi = min(_beta(c, r - 1), l - _beta(c - 1, r - 1) - 1)
+    return min(max(hi if hi >= lo else lo, 1), l - 1)
+
+
+def _first_descent_steps(nsteps, nsnap):
+    """Forward steps of the reverse sweep's first descent, i.e. what a
+    recorded forward saves the sweep."""
     a, s, done = 0, nsnap - 1, 0
     while nsteps - a > 1 and s > 0:
-        l = nsteps - a
-        t = 1
-        while comb(s + t, s) < l:
-            t += 1
-        m = min(max(int(l - comb(s - 1 + t, s - 1)), 1), l - 1)
+        m = _split(nsteps - a, s)
         done, a, s = done + m, a + m, s - 1
     return done
+
+
+def _least_forward_steps(l, c):
+    """The least number of forward steps that reverses l steps keeping c states
+    (the first included), r l - beta(c+1, r-1) for the least r with
+    beta(c, r) >= l (Griewank and Walther, Algorithm 799: revolve, ACM TOMS 26 (2000))."""
+    r = 1
+    while _beta(c, r) < l:
+        r += 1
+    return r * l - _beta(c + 1, r - 1)
 
 
 def test_recorded_forward_rollout(gpu_lib):
@@ -175,6 +196,7 @@ def test_recorded_forward_rollout(gpu_lib):
             nf_rec = run.adjoint_rollout(0, dts, nsnap, d_n, d_0)
             assert np.array_equal(run.field_download(d_0), plain), f"nsnap={nsnap}"
             assert nf_rec == nf_plain - _first_descent_steps(nsteps, nsnap), (nsnap, nf_rec, nf_plain)
+            assert nf_plain == _least_forward_steps(nsteps, nsnap), (nsnap, nf_plain)
             assert np.array_equal(run.download(0)[mask], U0[mask])  # the starting state is back
             print(f"nsnap={nsnap}: {nf_plain} recomputed steps plain, {nf_rec} after a recorded forward")
         # a recording made from another starting state, or for other dts, is not used

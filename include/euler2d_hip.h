@@ -222,9 +222,10 @@ int e2d_adjoint_step(e2d_ctx *ctx, int buf, double dt, const double *d_lam_in, d
 
 /* Gradient of an nsteps rollout from the state in buffer nStep0 % 2 with the
  * frozen dt sequence h_dts[0..nsteps-1]: lam_0 = (dU_n / dU_0)^T lam_n.
- * Binomial checkpointing (recursive treeverse with Griewank's split) keeps at
- * most nsnap state snapshots in device memory; *n_forward returns the number
- * of forward steps recomputed.  On return the starting state is back in its
+ * Binomial checkpointing (Griewank and Walther's revolve schedule: the least
+ * number of forward steps for nsnap kept states) keeps at most nsnap state
+ * snapshots in device memory; *n_forward returns the number of forward steps
+ * recomputed.  On return the starting state is back in its
  * buffer. */
 int e2d_adjoint_rollout(e2d_ctx *ctx, int nStep0, int nsteps, const double *h_dts, int nsnap,
                         const double *d_lam_n, double *d_lam_0, int64_t *n_forward);

@@ -328,7 +328,7 @@ static int waves_per_simd(const void* kernel) {
 }
 
 // Segment plan of a fused launch over nstrips strips.  Long segments cover
-// about 80% of each strip's rows, in as many segments per strip as fill k whole
+// about 80% (FP64) or 88% (FP32, mixed) of each strip's rows, in as many segments per strip as fill k whole
 // rounds of the device's resident wavefronts (k the least that keeps them
 // within 40 rows); short ones of a quarter of their length cover the rest.
 // Every long segment then starts in the first k rounds, and what is left when
@@ -349,10 +349,15 @@ static int plan_segments(StepArgs<P>& a, int nstrips, int ny, int wps, int resid
   const bool forced = env_int("E2D_FUSED_SEG_ROWS", 0) >= 1 || env_int("E2D_FUSED_LONG_PCT", -1) >= 0;
   const int round = resident / std::max(1, nstrips);  // long segments per strip in one round
   if (!forced && round >= 1) {
-    const double rows80 = 0.8 * ny;
+    // the long segments' share of the rows: 80% for the FP64 kernels (3
+    // resident wavefronts per SIMD), 88% for the float ones (4-5): measured
+    // at 2048^2, FP64 210-213 us at 80% against 214-218 at 76, 84 and 88%;
+    // FP32 132.5 at 88% against 137 at 80%, mixed 104.5 against 107
+    const int cover = env_int("E2D_FUSED_COVER_PCT", wps >= 4 ? 88 : 80);
+    const double long_rows = std::min(100, std::max(10, cover)) / 100.0 * ny;
     int k = 1;
-    while (rows80 / (k * round) > 40.0) ++k;
-    R = static_cast<int>(rows80 / (k * round) + 0.5);
+    while (long_rows / (k * round) > 40.0) ++k;
+    R = static_cast<int>(long_rows / (k * round) + 0.5);
     L = k * round;
     if (R < 8) R = 0;  // small grids: the percentage plan below
     while (R > 0 && L * R > ny) --R;

@@ -971,8 +971,9 @@ static int adj_env(const char* name, int dflt) {
 // Segment plan of k_adj_march_gather over nstrips strips of `rows` completed
 // rows: long segments of about rows/170 rows (within [4, 16];
 // E2D_ADJOINT_SEG_ROWS overrides) over the lower 75% of the rows
-// (E2D_ADJOINT_LONG_PCT), short ones of a quarter of that
-// (E2D_ADJOINT_SHORT_ROWS) over the rest, dealt to the XCDs in chunks of 16
+// (E2D_ADJOINT_LONG_PCT), short ones of a third of that
+// (E2D_ADJOINT_SHORT_ROWS; 4 rows against 3: FP64 720 -> 715 us, FP32
+// 441 -> 430 us, mixed 386 -> 380 us) over the rest, dealt to the XCDs in chunks of 16
 // segments (E2D_ADJOINT_CHUNK), so every XCD ends on short segments.
 // Measured at 2048^2 with a dense lam, us per adjoint step against uniform
 // 6-row segments: FP64 765 -> 721, FP32 481 -> 437, mixed 417 -> 384; 16-row
@@ -985,7 +986,7 @@ int plan_adjoint_segments(AdjArgs<P>& a, int nstrips, int rows) {
     R = static_cast<int>(rows / 170.0 + 0.5);
     R = R < 4 ? 4 : (R > 16 ? 16 : R);
   }
-  const int r = adj_env("E2D_ADJOINT_SHORT_ROWS", R / 4);
+  const int r = adj_env("E2D_ADJOINT_SHORT_ROWS", R / 3);
   const int pct = adj_env("E2D_ADJOINT_LONG_PCT", 75);
   const int L = pct >= 100 ? (rows + R - 1) / R
                            : static_cast<int>(static_cast<int64_t>(rows) * (pct < 0 ? 0 : pct) / 100 / R);

@@ -328,9 +328,10 @@ static int waves_per_simd(const void* kernel) {
 }
 
 // Segment plan of a fused launch over nstrips strips.  Long segments cover
-// about 80% (FP64) or 88% (FP32, mixed) of each strip's rows, in as many segments per strip as fill k whole
-// rounds of the device's resident wavefronts (k the least that keeps them
-// within 40 rows); short ones of a quarter of their length cover the rest.
+// about 80% (FP64) or 88% (FP32, mixed) of each strip's rows, in as many
+// segments per strip as fill k whole rounds of the device's resident
+// wavefronts (k the least that keeps them within 40 rows); short ones of a
+// quarter of their length cover the rest.
 // Every long segment then starts in the first k rounds, and what is left when
 // they end is short work, so no long segment runs alone at the end of the
 // launch.  Measured on MI355X (us per step, dt on the device): 2048^2 FP64

@@ -619,9 +619,17 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
 #endif
 #pragma unroll
   for (int v = 0; v < 4; ++v) qz(v) = acc(v) = ypf(v) = T(0);
+  // lam of this lane's cell in rows r and r-1, zero outside the interior: one
+  // load per row; the x-faces take the left column's by a lane shift
+  double lam_r[4], lam_p[4] = {0.0, 0.0, 0.0, 0.0};
 
   for (int r = ra - 1; r <= rb; ++r) {
     prim_at(a, cl, row(r + 1), qc);
+    {
+      const bool in = interior(g, ci, r);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) lam_r[v] = in ? a.lam[at(g, ci, r, v)] : 0.0;
+    }
     T qxm[4], qxp[4], dqx[4], dqy[4], s[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -642,9 +650,8 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
         bool any = false;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          double x = 0.0;
-          if (interior(g, ci, r)) x += a.lam[at(g, ci, r, v)];
-          if (interior(g, ci, r - 1)) x -= a.lam[at(g, ci, r - 1, v)];
+          // 0 + lam and x - 0 give the bits of leaving out the zero terms
+          const double x = (0.0 + lam_r[v]) - lam_p[v];
           fb[v] = static_cast<T>(x * static_cast<double>(dtdyU));
           any |= (x != 0.0);
         }
@@ -743,14 +750,15 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
         face_state(a.h, qb, dqx, s, dtdx, false, xr);
 #pragma unroll
         for (int v = 0; v < 4; ++v) xl[v] = from_left<true>(xmax_own[v]);
+        double lam_l[4];  // lam_r of column ci-1
+#pragma unroll
+        for (int v = 0; v < 4; ++v) lam_l[v] = from_left<true>(lam_r[v]);
         if (xface && r >= 2 && r <= g.ny + 1) {
           T fb[4];
           bool any = false;
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            double x = 0.0;
-            if (interior(g, ci, r)) x += a.lam[at(g, ci, r, v)];
-            if (interior(g, ci - 1, r)) x -= a.lam[at(g, ci - 1, r, v)];
+            const double x = (0.0 + lam_r[v]) - lam_l[v];
             fb[v] = static_cast<T>(x * static_cast<double>(dtdxU));
             any |= (x != 0.0);
           }
@@ -771,6 +779,7 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
       qz(v) = qa[v];
       qa[v] = qb[v];
       qb[v] = qc[v];
+      lam_p[v] = lam_r[v];
     }
   }
 }

@@ -27,6 +27,7 @@
 
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "e2d_adjoint.hpp"
 #include "e2d_common.hpp"
@@ -557,6 +558,24 @@ __device__ __forceinline__ void finish_cell(const AdjArgs<P>& a, int ci, int row
   }
 }
 
+// finish_cell with lam of the cell already in registers (zero for ghost cells,
+// where it is not used)
+template <class P>
+__device__ __forceinline__ void finish_cell_lam(const AdjArgs<P>& a, int ci, int row, const typename P::T qb[4],
+                                                const double lam[4]) {
+  using T = typename P::T;
+  const Grid& g = a.g;
+  T u[4], ub[4] = {T(0), T(0), T(0), T(0)};
+  load4(a.u, g, ci, row, u);
+  cons_to_prim_adj<typename P::M>(a.h, u, qb, ub);
+  if (interior(g, ci, row)) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) a.out[at(g, ci, row, v)] = lam[v] + static_cast<double>(ub[v]);
+  } else {
+    store4(a.ub, g, ci, row, ub);
+  }
+}
+
 template <class P>
 __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march_gather(const AdjArgs<P> a) {
   using T = typename P::T;
@@ -622,9 +641,22 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
   // lam of this lane's cell in rows r and r-1, zero outside the interior: one
   // load per row; the x-faces take the left column's by a lane shift
   double lam_r[4], lam_p[4] = {0.0, 0.0, 0.0, 0.0};
+  T un[4];  // state of row r+1, loaded an iteration ahead
+  load4(a.u, g, cl, row(ra), un);
+  // lam of row r-2 for finish_cell, carried where it keeps the kernel within
+  // 168 VGPRs (3 waves per SIMD): the mixed instantiation, 167 VGPRs.  FP32
+  // would take 173 (2 waves) and FP64 more scratch spills, both slower.
+  constexpr bool kLamPP = std::is_same<M, FastMath>::value;
+  double lam_pp[4] = {0.0, 0.0, 0.0, 0.0};
 
   for (int r = ra - 1; r <= rb; ++r) {
-    prim_at(a, cl, row(r + 1), qc);
+    {
+      T uc[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) uc[v] = un[v];
+      load4(a.u, g, cl, row(r + 2), un);
+      cons_to_prim<T, M>(a.h, uc, qc);
+    }
     {
       const bool in = interior(g, ci, r);
 #pragma unroll
@@ -715,7 +747,10 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
             T qfin[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) qfin[v] = acc(v) + ymf[v];
-            finish_cell(a, ci, ks - 1, qfin);
+            if constexpr (kLamPP)
+              finish_cell_lam(a, ci, ks - 1, qfin, lam_pp);
+            else
+              finish_cell(a, ci, ks - 1, qfin);
           }
           if (k == ra) {
 #pragma unroll
@@ -779,6 +814,7 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
       qz(v) = qa[v];
       qa[v] = qb[v];
       qb[v] = qc[v];
+      if constexpr (kLamPP) lam_pp[v] = lam_p[v];
       lam_p[v] = lam_r[v];
     }
   }

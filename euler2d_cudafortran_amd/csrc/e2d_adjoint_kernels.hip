@@ -294,9 +294,25 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
   // row r-1, carried: slopes, trace sources and the bars of its xmin, xmax, ymin face states
   // (its primitives are qa)
   T pdx[4], pdy[4], ps[4], pbxm[4], pbxp[4], pbym[4];
+  // lam of this lane's cell in rows r and r-1, zero outside the interior: one
+  // load per row; the x-faces take the left column's by a lane shift
+  double lam_r[4], lam_p[4] = {0.0, 0.0, 0.0, 0.0};
+  T un[4];  // state of row r+1, loaded an iteration ahead
+  load4(a.u, g, cl, row(ra), un);
 
   for (int r = ra - 1; r <= rb; ++r) {
-    prim_at(a, cl, row(r + 1), qc);
+    {
+      T uc[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) uc[v] = un[v];
+      load4(a.u, g, cl, row(r + 2), un);
+      cons_to_prim<T, M>(a.h, uc, qc);
+    }
+    {
+      const bool in = interior(g, ci, r);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) lam_r[v] = in ? a.lam[at(g, ci, r, v)] : 0.0;
+    }
     T qxm[4], qxp[4], dqx[4], dqy[4], s[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -317,9 +333,8 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
         bool any = false;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          double x = 0.0;
-          if (interior(g, ci, r)) x += a.lam[at(g, ci, r, v)];
-          if (interior(g, ci, r - 1)) x -= a.lam[at(g, ci, r - 1, v)];
+          // 0 + lam and x - 0 give the bits of leaving out the zero terms
+          const double x = (0.0 + lam_r[v]) - lam_p[v];
           fb[v] = static_cast<T>(x * static_cast<double>(dtdyU));
           any |= (x != 0.0);
         }
@@ -356,14 +371,15 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
         face_state(a.h, qb, dqx, s, dtdx, false, xr);
 #pragma unroll
         for (int v = 0; v < 4; ++v) xl[v] = from_left<true>(xmax_own[v]);
+        double lam_l[4];  // lam_r of column ci-1
+#pragma unroll
+        for (int v = 0; v < 4; ++v) lam_l[v] = from_left<true>(lam_r[v]);
         if (xface && r >= 2 && r <= g.ny + 1) {
           T fb[4];
           bool any = false;
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            double x = 0.0;
-            if (interior(g, ci, r)) x += a.lam[at(g, ci, r, v)];
-            if (interior(g, ci - 1, r)) x -= a.lam[at(g, ci - 1, r, v)];
+            const double x = (0.0 + lam_r[v]) - lam_l[v];
             fb[v] = static_cast<T>(x * static_cast<double>(dtdxU));
             any |= (x != 0.0);
           }
@@ -383,6 +399,7 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
       pbym[v] = bym[v];
       qa[v] = qb[v];
       qb[v] = qc[v];
+      lam_p[v] = lam_r[v];
     }
   }
   (void)zero4;

@@ -122,6 +122,12 @@ def cpu_all_cores(nx: int, steps: int) -> dict:
                       f"wall {wall:.1f} s"}
 
 
+def state_bytes(e2d, prec) -> int:
+    """Bytes per stored conserved variable: 8 for the FP64 state of fp64 and
+    mixed, 4 for the FP32 state of fp32 and fp32_native."""
+    return 8 if prec in (e2d.PREC_FP64, e2d.PREC_MIXED) else 4
+
+
 def shutil_which(name):
     import shutil
     return shutil.which(name)
@@ -134,8 +140,9 @@ def variant(e2d, nx, impl, prec, steps, warmup):
     finally:
         run.close()
     # algorithmic HBM bytes per step: the 4 conserved variables read and
-    # written once per cell, in the storage type (8 B in FP64, 4 B in FP32 and mixed)
-    alg = nx * nx * 4 * 2 * (8 if prec == e2d.PREC_FP64 else 4)
+    # written once per cell, in the storage type (8 B for the FP64 state of
+    # fp64 and mixed, 4 B for the FP32 state of fp32 and fp32_native)
+    alg = nx * nx * 4 * 2 * state_bytes(e2d, prec)
     return {"mcell_per_s": round(nx * nx * steps / wall / 1e6, 1), "steps": steps,
             "ms_per_step": round(1e3 * wall / steps, 4), "kernel_ms": round(kms, 4),
             "hbm_frac_algorithmic": round(alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
@@ -164,19 +171,24 @@ def host_loop(e2d, nx, steps, warmup=10):
 def precision_error(e2d, nx, steps):
     """FP32 variants against the FP64 run on the bumps problem: relative L1 and
     max-abs error per conserved variable after `steps` steps."""
-    fields = {}
-    for prec in (e2d.PREC_FP64, e2d.PREC_FP32, e2d.PREC_MIXED):
+    def final(prec):
         run = make_run(e2d, nx, 0, prec, problem="bumps")
         run.run_steps(steps)
-        fields[prec] = run.download(run.current())[2:nx + 2, 2:nx + 2, :]
+        u = run.download(run.current())[2:nx + 2, 2:nx + 2, :]
+        sums = run.sums(run.current())
         run.close()
-    ref = fields[e2d.PREC_FP64]
+        return u, sums
+
+    ref, ref_sums = final(e2d.PREC_FP64)
     out = {}
-    for prec, name in ((e2d.PREC_FP32, "fp32"), (e2d.PREC_MIXED, "mixed")):
-        d = fields[prec] - ref
+    for name in ("fp32", "fp32_native", "mixed"):
+        u, sums = final(e2d.PRECISIONS[name])
+        d = u - ref
+        del u
         out[name] = {
             "rel_l1": [float(np.abs(d[..., v]).sum() / np.abs(ref[..., v]).sum()) for v in range(4)],
             "linf": [float(np.abs(d[..., v]).max()) for v in range(4)],
+            "mass_energy_sums_rel": [float(abs(sums[v] - ref_sums[v]) / abs(ref_sums[v])) for v in (0, 1)],
         }
     return out
 
@@ -225,7 +237,7 @@ def adjoint_metrics(e2d):
     Forward times are the step launches alone (dt frozen), gradient times the
     whole checkpointed reverse sweep including recomputation."""
     out = {}
-    for name, prec in (("fp64", e2d.PREC_FP64), ("fp32", e2d.PREC_FP32), ("mixed", e2d.PREC_MIXED)):
+    for name, prec in e2d.PRECISIONS.items():
         run = make_run(e2d, 2048, 0, prec)
         run.run_steps(100)
         run.synchronize()
@@ -287,7 +299,7 @@ def adjoint_metrics(e2d):
 
     dts = record_dts(nx, nsteps)
     c5, grads = {}, {}
-    for name, prec in (("fp64", e2d.PREC_FP64), ("fp32", e2d.PREC_FP32)):
+    for name, prec in (("fp64", e2d.PREC_FP64), ("fp32", e2d.PREC_FP32), ("mixed", e2d.PREC_MIXED)):
         t_f, t_a, nf, g, uT = _rollout(e2d, nx, prec, dts, nsnap, misfit_lam)
         grads[name] = g[2:nx + 2, 2:nx + 2, :]
         c5[name] = {"forward_s": round(t_f, 4), "gradient_s": round(t_a, 4),
@@ -308,10 +320,11 @@ def adjoint_metrics(e2d):
             "forward_s": round(t_f, 4), "gradient_s": round(t_a, 4), "gradient_over_forward": round(t_a / t_f, 3),
             "forward_steps_recomputed": nf,
             "same_gradient_as_45_snapshots": bool(np.array_equal(g[2:nx + 2, 2:nx + 2, :], grads["fp64"]))}
-    d = grads["fp32"] - grads["fp64"]
-    c5["fp32_gradient_rel_l2_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(grads["fp64"]))
-    c5["fp32_gradient_rel_l2_vs_fp64_per_variable"] = [
-        float(np.linalg.norm(d[..., v]) / np.linalg.norm(grads["fp64"][..., v])) for v in range(4)]
+    for name in ("fp32", "mixed"):
+        d = grads[name] - grads["fp64"]
+        c5[f"{name}_gradient_rel_l2_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(grads["fp64"]))
+        c5[f"{name}_gradient_rel_l2_vs_fp64_per_variable"] = [
+            float(np.linalg.norm(d[..., v]) / np.linalg.norm(grads["fp64"][..., v])) for v in range(4)]
     out["rollout_2048_2000steps_45snap_misfit"] = c5
     return out
 
@@ -323,7 +336,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nx", type=int, default=2048)
     ap.add_argument("--impl", type=int, default=0)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "mixed"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "fp32_native", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=4)
@@ -397,14 +410,13 @@ def main(argv=None):
     for _ in range(reps):
         run.copy_from_buffer(0, dst)
     copy_s = (time.perf_counter() - t0) / reps
-    buf_bytes = run.device_buffer(0)[1] * (nx + 4) * 4 * (8 if prec == e2d.PREC_FP64 else 4)
+    buf_bytes = run.device_buffer(0)[1] * (nx + 4) * 4 * state_bytes(e2d, prec)
     copy_gbs = 2 * buf_bytes / copy_s / 1e9
     run.close()
     cells = nx * nx
     value = cells * args.steps / wall / 1e6
     ms_per_step = 1e3 * wall / args.steps
-    state_bytes = 8 if prec == e2d.PREC_FP64 else 4
-    alg_bytes = cells * 4 * state_bytes * 2  # read + write of 4 conserved variables
+    alg_bytes = cells * 4 * state_bytes(e2d, prec) * 2  # read + write of 4 conserved variables
     achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
     traffic, valu = None, None
     if TRAFFIC_FILE.exists():
@@ -428,9 +440,11 @@ def main(argv=None):
             "fp64_materialised_2048": (2048, 1, e2d.PREC_FP64, 200),
             "fp64_flux_store_2048": (2048, 2, e2d.PREC_FP64, 200),
             "fp32_fused_2048": (2048, 0, e2d.PREC_FP32, 500),
+            "fp32_native_fused_2048": (2048, 0, e2d.PREC_FP32_NATIVE, 500),
             "mixed_fused_2048": (2048, 0, e2d.PREC_MIXED, 500),
             "fp64_fused_4096": (4096, 0, e2d.PREC_FP64, 200),
             "fp32_fused_4096": (4096, 0, e2d.PREC_FP32, 500),
+            "fp32_native_fused_4096": (4096, 0, e2d.PREC_FP32_NATIVE, 500),
             "mixed_fused_4096": (4096, 0, e2d.PREC_MIXED, 500),
         }.items():
             _log(f"[bench] variant {name}")
@@ -463,7 +477,7 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "mixed": "f32+f64"}[args.precision],
+        "dtype": {"fp64": "f64", "fp32": "f32", "fp32_native": "f32", "mixed": "f32+f64"}[args.precision],
         "data": "synthetic",
         "config": {"workload": f"euler2d {nx}x{nx} {args.precision}, implementationVersion {args.impl}, "
                                "diagonal discontinuity, gamma 1.4, CFL 0.8, minmod, reflective walls",

@@ -1132,15 +1132,16 @@ hipError_t adjoint_step(const e2d_params& p, const DevBuffers& b, int buf, doubl
 
 }  // namespace
 
-size_t adjoint_scratch_elem_bytes(int precision) { return precision == E2D_PREC_FP64 ? sizeof(double) : sizeof(float); }
+// the scratch arrays hold values of the step's arithmetic type
+size_t adjoint_scratch_elem_bytes(int precision) {
+  return with_precision(precision, [](auto pol) { return sizeof(typename decltype(pol)::T); });
+}
 
 hipError_t launch_adjoint_step(const e2d_params& p, const DevBuffers& b, int buf, double dt, const double* lam_in,
                                double* lam_out, const AdjointScratch& s, hipStream_t st) {
-  switch (p.precision) {
-    case E2D_PREC_FP64: return adjoint_step<P64>(p, b, buf, dt, lam_in, lam_out, s, st);
-    case E2D_PREC_FP32: return adjoint_step<P32>(p, b, buf, dt, lam_in, lam_out, s, st);
-    default: return adjoint_step<PMix>(p, b, buf, dt, lam_in, lam_out, s, st);
-  }
+  return with_precision(p.precision, [&](auto pol) {
+    return adjoint_step<decltype(pol)>(p, b, buf, dt, lam_in, lam_out, s, st);
+  });
 }
 
 }  // namespace e2d

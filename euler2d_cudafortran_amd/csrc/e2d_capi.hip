@@ -53,7 +53,8 @@ int validate(const e2d_params* p) {
                 "The implementation version parameter should be 0, 1 or 2 (got %d); check the namelist, section OTHER",
                 p->implementation_version);
   if (p->precision < E2D_PREC_FP64 || p->precision > E2D_PREC_MIXED)
-    return fail(E2D_ERR_ARG, "precision must be 0 (fp64), 1 (fp32) or 2 (mixed), got %d", p->precision);
+    return fail(E2D_ERR_ARG, "precision must be 0 (fp64), 1 (fp32), 2 (fp32_native) or 3 (mixed), got %d",
+                p->precision);
   if (p->riemann_solver != E2D_RIEMANN_APPROX && p->riemann_solver != E2D_RIEMANN_HLLC)
     return fail(E2D_ERR_ARG, "riemann_solver must be 0 (approx) or 2 (hllc), got %d", p->riemann_solver);
   if (p->niter_riemann < 0) return fail(E2D_ERR_ARG, "niter_riemann must be >= 0");

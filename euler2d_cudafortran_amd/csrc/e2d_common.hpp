@@ -26,7 +26,22 @@ struct Prec {
 };
 using P64 = Prec<double, double, double, RangedMath, RangedMath>;
 using P32 = Prec<float, float, float, RangedMath, RangedMath>;
-using PMix = Prec<float, float, double, FastMath, RangedMath>;
+// FP32 state; FP32 flux arithmetic with the native reciprocal and square
+// root; the update added in FP64 and rounded to the FP32 state
+using PNative = Prec<float, float, double, FastMath, RangedMath>;
+// FP64 state and update, FP32 slope / trace / Riemann arithmetic
+using PMix = Prec<double, float, double, RangedMath, RangedMath>;
+
+// f(P{}) for the policy P of an E2D_PREC_* code (validated by the caller)
+template <class F>
+inline auto with_precision(int precision, F&& f) {
+  switch (precision) {
+    case E2D_PREC_FP64: return f(P64{});
+    case E2D_PREC_FP32: return f(P32{});
+    case E2D_PREC_FP32_NATIVE: return f(PNative{});
+    default: return f(PMix{});
+  }
+}
 
 // Derived parameters in precision T, as initHydroParameters computes them
 // (serial_ref/m_run.f90:111-124) plus the cmpflx constant entho (m_utils.f90:521).

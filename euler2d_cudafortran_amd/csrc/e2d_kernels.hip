@@ -420,11 +420,7 @@ static hipError_t fused_step(const e2d_params& p, const DevBuffers& b, const Ste
 }
 
 hipError_t launch_fused_step(const e2d_params& p, const DevBuffers& b, const StepCfg& c, hipStream_t s) {
-  switch (p.precision) {
-    case E2D_PREC_FP64: return fused_step<P64>(p, b, c, s);
-    case E2D_PREC_FP32: return fused_step<P32>(p, b, c, s);
-    default: return fused_step<PMix>(p, b, c, s);
-  }
+  return with_precision(p.precision, [&](auto pol) { return fused_step<decltype(pol)>(p, b, c, s); });
 }
 
 // ---------------------------------------------------------------------------
@@ -524,11 +520,7 @@ static hipError_t boundaries(const e2d_params& p, const DevBuffers& b, int buf, 
 }
 
 hipError_t launch_boundaries(const e2d_params& p, const DevBuffers& b, int buf, hipStream_t s) {
-  switch (p.precision) {
-    case E2D_PREC_FP64: return boundaries<P64>(p, b, buf, s);
-    case E2D_PREC_FP32: return boundaries<P32>(p, b, buf, s);
-    default: return boundaries<PMix>(p, b, buf, s);
-  }
+  return with_precision(p.precision, [&](auto pol) { return boundaries<decltype(pol)>(p, b, buf, s); });
 }
 
 // ---------------------------------------------------------------------------
@@ -575,11 +567,7 @@ static hipError_t max_invdt(const e2d_params& p, const DevBuffers& b, int buf, i
 }
 
 hipError_t launch_max_invdt(const e2d_params& p, const DevBuffers& b, int buf, int slot, hipStream_t s) {
-  switch (p.precision) {
-    case E2D_PREC_FP64: return max_invdt<P64>(p, b, buf, slot, s);
-    case E2D_PREC_FP32: return max_invdt<P32>(p, b, buf, slot, s);
-    default: return max_invdt<PMix>(p, b, buf, slot, s);
-  }
+  return with_precision(p.precision, [&](auto pol) { return max_invdt<decltype(pol)>(p, b, buf, slot, s); });
 }
 
 __global__ void k_fill(double* dst, int n, double v) {
@@ -902,11 +890,7 @@ static hipError_t materialised_step(const e2d_params& p, const DevBuffers& b, co
 
 hipError_t launch_materialised_step(const e2d_params& p, const DevBuffers& b, const StepCfg& c,
                                     hipStream_t s) {
-  switch (p.precision) {
-    case E2D_PREC_FP64: return materialised_step<P64>(p, b, c, s);
-    case E2D_PREC_FP32: return materialised_step<P32>(p, b, c, s);
-    default: return materialised_step<PMix>(p, b, c, s);
-  }
+  return with_precision(p.precision, [&](auto pol) { return materialised_step<decltype(pol)>(p, b, c, s); });
 }
 
 // ---------------------------------------------------------------------------
@@ -986,11 +970,8 @@ static hipError_t init_problem(const e2d_params& p, const DevBuffers& b, int pro
 
 hipError_t launch_init_problem(const e2d_params& p, const DevBuffers& b, int problem, const double* bumps,
                                hipStream_t s) {
-  switch (p.precision) {
-    case E2D_PREC_FP64: return init_problem<P64>(p, b, problem, bumps, s);
-    case E2D_PREC_FP32: return init_problem<P32>(p, b, problem, bumps, s);
-    default: return init_problem<PMix>(p, b, problem, bumps, s);
-  }
+  return with_precision(p.precision,
+                        [&](auto pol) { return init_problem<decltype(pol)>(p, b, problem, bumps, s); });
 }
 
 template <class S>
@@ -1110,12 +1091,16 @@ hipError_t launch_pack_field(const e2d_params& p, const DevBuffers& b, const dou
   return hipGetLastError();
 }
 
-size_t state_elem_bytes(int precision) { return precision == E2D_PREC_FP64 ? 8 : 4; }
+size_t state_elem_bytes(int precision) {
+  return with_precision(precision, [](auto pol) { return sizeof(typename decltype(pol)::S); });
+}
 
+// q, slopes and face states in the flux arithmetic type, fx and fy in the update type
 size_t stage_elem_bytes(int precision, int stage) {
-  if (precision == E2D_PREC_FP64) return 8;
-  if (precision == E2D_PREC_FP32) return 4;
-  return (stage == E2D_STAGE_FX || stage == E2D_STAGE_FY) ? 8 : 4;  // mixed: fluxes kept in FP64
+  return with_precision(precision, [&](auto pol) {
+    using P = decltype(pol);
+    return (stage == E2D_STAGE_FX || stage == E2D_STAGE_FY) ? sizeof(typename P::U) : sizeof(typename P::T);
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -1185,17 +1170,11 @@ __global__ __launch_bounds__(256) void k_pointwise(const Hydro<typename P::T> h,
 
 hipError_t launch_pointwise(const e2d_params& p, int func, int n, const double* d_x, double* d_y, hipStream_t s) {
   const dim3 grid((n + 255) / 256);
-  switch (p.precision) {
-    case E2D_PREC_FP64:
-      hipLaunchKernelGGL(k_pointwise<P64>, grid, dim3(256), 0, s, make_hydro<double>(p), func, n, d_x, d_y);
-      break;
-    case E2D_PREC_FP32:
-      hipLaunchKernelGGL(k_pointwise<P32>, grid, dim3(256), 0, s, make_hydro<float>(p), func, n, d_x, d_y);
-      break;
-    default:
-      hipLaunchKernelGGL(k_pointwise<PMix>, grid, dim3(256), 0, s, make_hydro<float>(p), func, n, d_x, d_y);
-      break;
-  }
+  with_precision(p.precision, [&](auto pol) {
+    using P = decltype(pol);
+    hipLaunchKernelGGL(k_pointwise<P>, grid, dim3(256), 0, s, make_hydro<typename P::T>(p), func, n, d_x, d_y);
+    return 0;
+  });
   return hipGetLastError();
 }
 

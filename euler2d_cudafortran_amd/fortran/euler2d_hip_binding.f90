@@ -8,7 +8,8 @@ module euler2d_hip
   implicit none
 
   integer(c_int), parameter :: E2D_IMPL_FUSED = 0, E2D_IMPL_MATERIALISED = 1, E2D_IMPL_FLUX_STORE = 2
-  integer(c_int), parameter :: E2D_PREC_FP64 = 0, E2D_PREC_FP32 = 1, E2D_PREC_MIXED = 2
+  integer(c_int), parameter :: E2D_PREC_FP64 = 0, E2D_PREC_FP32 = 1, E2D_PREC_FP32_NATIVE = 2, &
+       E2D_PREC_MIXED = 3
   integer(c_int), parameter :: E2D_RIEMANN_APPROX = 0, E2D_RIEMANN_HLLC = 2
   integer(c_int), parameter :: E2D_PROBLEM_IMPLODE = 0, E2D_PROBLEM_DIAG = 1, E2D_PROBLEM_BUMPS = 2
   integer(c_int), parameter :: E2D_TIMER_GODUNOV = 0, E2D_TIMER_BOUNDARIES = 1, E2D_TIMER_DT = 2

@@ -3,7 +3,7 @@
 !!
 !! Namelist groups &RUN, &MESH, &HYDRO as in the reference and &OTHER with
 !! implementationVersion (0 fused, 1 materialised, 2 flux-store) plus this implementation's
-!! options: precision ('fp64' | 'fp32' | 'mixed'), problem ('implode' | 'diag'
+!! options: precision ('fp64' | 'fp32' | 'fp32_native' | 'mixed'), problem ('implode' | 'diag'
 !! | 'bumps'), riemann_solver (0 two-shock, 2 HLLC), seed, device_loop
 !! (keep dt on the device between outputs), async_output (with device_loop:
 !! launch the steps up to the next output before writing the current one, so
@@ -117,10 +117,12 @@ contains
        params%precision = E2D_PREC_FP64
     case ('fp32')
        params%precision = E2D_PREC_FP32
+    case ('fp32_native')
+       params%precision = E2D_PREC_FP32_NATIVE
     case ('mixed')
        params%precision = E2D_PREC_MIXED
     case default
-       write(*,*) 'precision should be fp64, fp32 or mixed; got ', trim(precision)
+       write(*,*) 'precision should be fp64, fp32, fp32_native or mixed; got ', trim(precision)
        stop
     end select
   end subroutine initHydroParameters

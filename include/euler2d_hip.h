@@ -45,9 +45,13 @@ extern "C" {
 #define E2D_IMPL_FLUX_STORE 2   /* q and fluxes stored in HBM, slopes and face states in registers */
 
 /* precision */
-#define E2D_PREC_FP64 0   /* FP64 state and arithmetic (serial_ref built with fp_kind = dp_kind) */
-#define E2D_PREC_FP32 1   /* FP32 state and arithmetic (serial_ref default kind) */
-#define E2D_PREC_MIXED 2  /* FP32 state and flux arithmetic, FP64 update/dt/sums */
+#define E2D_PREC_FP64 0        /* FP64 state and arithmetic (serial_ref built with fp_kind = dp_kind) */
+#define E2D_PREC_FP32 1        /* FP32 state and arithmetic (serial_ref default kind) */
+#define E2D_PREC_FP32_NATIVE 2 /* FP32 state; FP32 flux arithmetic with the native reciprocal and square
+                                  root (about 1 ulp); update added in FP64 and rounded to the FP32 state;
+                                  FP64 dt and sums */
+#define E2D_PREC_MIXED 3       /* FP64 state and FP64 update, FP32 slope/trace/Riemann arithmetic
+                                  (correctly rounded division and square root), FP64 dt and sums */
 
 /* riemann_solver: serial_ref/m_constants.f90:21-23 */
 #define E2D_RIEMANN_APPROX 0

@@ -285,19 +285,26 @@ def test_sums_match_host(gpu_lib):
 
 
 # ---------------------------------------------------------------------------
-# mixed precision variant: error against FP64 stated
+# float variants without a reference kind: error against FP64 stated
 
-def test_mixed_precision_against_fp64(gpu_lib):
+# relative L1 after 200 steps of the 256^2 diagonal problem, per variable;
+# measured on MI355X: fp32_native 1.07e-7 6.88e-8 3.02e-7 3.02e-7, mixed
+# 3.18e-8 3.35e-8 1.33e-7 1.33e-7; bounds three times those
+FLOAT_VARIANT_BOUNDS = {e2d.PREC_FP32_NATIVE: [3.2e-7, 2.1e-7, 9.1e-7, 9.1e-7],
+                        e2d.PREC_MIXED: [9.6e-8, 1e-7, 4e-7, 4e-7]}
+
+
+@pytest.mark.parametrize("prec", [e2d.PREC_FP32_NATIVE, e2d.PREC_MIXED])
+def test_float_variants_against_fp64(gpu_lib, prec):
     out = {}
-    for prec in (e2d.PREC_FP64, e2d.PREC_MIXED):
-        with e2d.HydroRun(_params(256, prec=prec)) as run:
+    for pr in (e2d.PREC_FP64, prec):
+        with e2d.HydroRun(_params(256, prec=pr)) as run:
             run.init("diag")
             run.make_boundaries(0)
             run.run_steps(200)
-            out[prec] = run.download(run.current())
-    a, b = out[e2d.PREC_MIXED], out[e2d.PREC_FP64]
+            out[pr] = run.download(run.current())
+    a, b = out[prec], out[e2d.PREC_FP64]
     i = slice(2, 258)
-    for v in range(4):
-        err = rel_l1(a[i, i, v], b[i, i, v])
-        print(f"mixed vs fp64, variable {v}: rel L1 {err:.3e}")
-        assert err < 1e-4
+    errs = [rel_l1(a[i, i, v], b[i, i, v]) for v in range(4)]
+    print(f"precision {prec} vs fp64, 256^2 diagonal problem, 200 steps: rel L1 {['%.3e' % e for e in errs]}")
+    assert all(e <= bnd for e, bnd in zip(errs, FLOAT_VARIANT_BOUNDS[prec])), errs

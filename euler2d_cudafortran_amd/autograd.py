@@ -28,6 +28,28 @@ def state_tensor(run: HydroRun, buf: int) -> torch.Tensor:
     return t
 
 
+def check_state_tensor(run: HydroRun, t: torch.Tensor, what: str):
+    """Raise ValueError unless `t` can be handed to the library as a whole FP64
+    state or field of `run`: a CUDA float64 tensor of shape (4, jsize, pitch)
+    (the device layout), and the run in precision 0 (fp64).  The library copies
+    state_bytes / field_bytes from such an address without knowing the tensor,
+    so anything smaller would be read or written past its end."""
+    if run.params.precision != PREC_FP64:
+        raise ValueError("the autograd harness needs precision 0 (fp64), "
+                         f"got {run.params.precision}")
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{what} must be a torch.Tensor, got {type(t).__name__}")
+    if t.dtype != torch.float64:
+        raise ValueError(f"{what} must be float64, got {t.dtype}")
+    _, pitch = run.device_buffer(0)
+    want = (4, run.params.jsize, pitch)
+    if tuple(t.shape) != want:
+        raise ValueError(f"{what} must have the device layout shape {want} (4, jsize, pitch), "
+                         f"got {tuple(t.shape)}")
+    if not t.is_cuda:
+        raise ValueError(f"{what} must be a CUDA tensor, got device {t.device}")
+
+
 def interior_mask(run: HydroRun, like: torch.Tensor) -> torch.Tensor:
     m = torch.zeros_like(like, dtype=torch.bool)
     m[:, 2:run.params.ny + 2, 2:run.params.nx + 2] = True
@@ -37,6 +59,7 @@ def interior_mask(run: HydroRun, like: torch.Tensor) -> torch.Tensor:
 def record_dts(run: HydroRun, u0: torch.Tensor, nsteps: int) -> list[float]:
     """Forward run from u0 with the reference's dt rule (compute_dt each
     step, running maximum reset first); returns the dt sequence."""
+    check_state_tensor(run, u0, "u0")
     run.invdt = 0.0
     torch.cuda.synchronize()
     run.copy_to_buffer(0, u0.data_ptr())
@@ -52,6 +75,7 @@ def record_dts(run: HydroRun, u0: torch.Tensor, nsteps: int) -> list[float]:
 class GodunovRollout(torch.autograd.Function):
     @staticmethod
     def forward(ctx, u0: torch.Tensor, run: HydroRun, dts, nsnap: int):
+        check_state_tensor(run, u0, "u0")
         u0 = u0.contiguous()
         torch.cuda.synchronize()
         run.copy_to_buffer(0, u0.data_ptr())
@@ -67,6 +91,7 @@ class GodunovRollout(torch.autograd.Function):
     def backward(ctx, grad_out: torch.Tensor):
         (u0,) = ctx.saved_tensors
         run = ctx.run
+        check_state_tensor(run, grad_out, "grad_out")
         g = grad_out.contiguous()
         lam0 = torch.zeros_like(g)
         torch.cuda.synchronize()

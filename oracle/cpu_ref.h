@@ -6,7 +6,10 @@
  * CPU restatement of the reference serial solver (pkestene/euler2d_cudaFortran,
  * serial_ref/), operation by operation and in source order, in two precisions:
  * suffix _f64 (the reference built with fp_kind = dp_kind) and _f32 (the
- * reference default, serial_ref/m_precision.f90:25).  Built with
+ * reference default, serial_ref/m_precision.f90:25), plus _f80 (long double)
+ * and _f128 (__float128): the FP64 map's arithmetic with a 64-bit and a
+ * 113-bit significand, used only to take finite differences above the FP64
+ * roundoff floor.  Built with
  * -ffp-contract=off so every operation rounds as the Fortran source reads.
  *
  * Arrays exchanged with the caller are Fortran column-major u(isize,jsize,4)
@@ -44,10 +47,14 @@ enum {
   int oracle_run_##S(void *h, int nStepmax, double tEnd, double *dts);                        \
   void oracle_get_##S(void *h, int which, double *out);                                       \
   void oracle_set_##S(void *h, int which, const double *in);                                  \
-  void oracle_kat_##S(void *h, int func, int n, const double *x, double *y);
+  void oracle_kat_##S(void *h, int func, int n, const double *x, double *y);                   \
+  int oracle_fd_##S(void *h, const double *u0, const double *dir, int nsteps, const double *dts, \
+                    double eps, const double *weight, int freeze_newton, double *out);
 
 ORACLE_DECLARE(f64)
 ORACLE_DECLARE(f32)
+ORACLE_DECLARE(f80)  /* x87 extended precision: finite differences of the FP64 map */
+ORACLE_DECLARE(f128) /* binary128: the same, where a smaller step is needed */
 
 /* thread count of the grid loops (n > 0 sets it); returns the count in use */
 int oracle_threads(int n);

@@ -26,6 +26,10 @@ typedef struct {
   R gamma0, gamma6, cfl, smallr, smallc, smallp, smallpp, dx, dy;
   R *u, *u2, *q, *fx, *fy;
   R invDt; /* SAVEd running max of compute_dt (serial_ref/m_run.f90:257) */
+  /* Newton iteration counts of the two-shock solver per step and face, for
+   * oracle_fd only: mode 1 records them, mode 2 replays them (0: unused) */
+  unsigned char *tape;
+  int tape_mode, tape_step;
 } FN(run);
 
 #define NBVAR 4
@@ -137,8 +141,10 @@ static inline void FN(cmpflx)(const FN(run) * h, const R g[4], R f[4]) {
 }
 
 /* riemann_2d: serial_ref/m_utils.f90:349-492.  Returns the Newton iteration
- * count (not part of the reference interface; used to grade test coverage). */
-static inline int FN(riemann)(const FN(run) * h, const R ql[4], const R qr[4], R g[4], R f[4]) {
+ * count (not part of the reference interface; used to grade test coverage).
+ * fixed < 0 keeps the reference's stop rule; fixed >= 0 runs exactly that
+ * many iterations instead (oracle_fd's frozen counts). */
+static inline int FN(riemann_n)(const FN(run) * h, const R ql[4], const R qr[4], R g[4], R f[4], int fixed) {
   const R rl = MAX2(ql[0], h->smallr);
   const R ul = ql[2];
   const R pl = MAX2(ql[1], rl * h->smallp);
@@ -156,7 +162,7 @@ static inline int FN(riemann)(const FN(run) * h, const R ql[4], const R qr[4], R
   R conv = (R)1;
 
   int iter = 0;
-  while (iter < h->niter_riemann && conv > LIT_F32(1e-6)) {
+  while (fixed >= 0 ? iter < fixed : (iter < h->niter_riemann && conv > LIT_F32(1e-6))) {
     R wwl = SQRT(cl * ((R)1 + h->gamma6 * (pold - pl) / pl));
     R wwr = SQRT(cr * ((R)1 + h->gamma6 * (pold - pr) / pr));
     R qlw = (R)2.0 * wwl * wwl * wwl / (wwl * wwl + cl);
@@ -209,6 +215,10 @@ static inline int FN(riemann)(const FN(run) * h, const R ql[4], const R qr[4], R
 
   FN(cmpflx)(h, g, f);
   return iter;
+}
+
+static inline int FN(riemann)(const FN(run) * h, const R ql[4], const R qr[4], R g[4], R f[4]) {
+  return FN(riemann_n)(h, ql, qr, g, f, -1);
 }
 
 /* riemann_2d_hllc: serial_ref/m_utils.f90:534-644 */
@@ -267,11 +277,24 @@ static inline void FN(hllc)(const FN(run) * h, const R ql[4], const R qr[4], R g
   f[3] = (f[0] > (R)0) ? f[0] * ql[3] : f[0] * qr[3];
 }
 
-static inline void FN(solve)(const FN(run) * h, const R ql[4], const R qr[4], R g[4], R f[4]) {
-  if (h->riemann_solver == 2)
+/* slot: this face's entry of the Newton-count tape, or NULL */
+static inline void FN(solve)(const FN(run) * h, const R ql[4], const R qr[4], R g[4], R f[4],
+                             unsigned char *slot) {
+  if (h->riemann_solver == 2) {
     FN(hllc)(h, ql, qr, g, f);
-  else
-    (void)FN(riemann)(h, ql, qr, g, f);
+  } else if (slot && h->tape_mode == 2) {
+    (void)FN(riemann_n)(h, ql, qr, g, f, *slot);
+  } else {
+    const int k = FN(riemann)(h, ql, qr, g, f);
+    if (slot && h->tape_mode == 1) *slot = (unsigned char)k;
+  }
+}
+
+/* tape entry of face dir (0: x, 1: y) of cell (i, j) in the current step */
+static inline unsigned char *FN(tape_slot)(const FN(run) * h, int i, int j, int dir) {
+  if (!h->tape || !h->tape_mode) return NULL;
+  return h->tape + ((((size_t)h->tape_step * (size_t)h->jsize + (size_t)(j - 1)) * (size_t)h->isize +
+                     (size_t)(i - 1)) * 2 + (size_t)dir);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -506,7 +529,7 @@ static void FN(compute_and_store_fluxes)(FN(run) * h, R dt) {
 
       FN(trace)(h, qLoc, dq, dq + 4, dtdx, dtdy, FACE_XMIN, qright);
       FN(trace)(h, qLocN, dqN, dqN + 4, dtdx, dtdy, FACE_XMAX, qleft);
-      FN(solve)(h, qleft, qright, qgdnv, flux_x);
+      FN(solve)(h, qleft, qright, qgdnv, flux_x, FN(tape_slot)(h, i, j, 0));
       for (int v = 0; v < NBVAR; ++v) AT(h->fx, i, j, v + 1) = flux_x[v] * dtdx;
 
       FN(load_q)(h, i, j - 1, qLocN);
@@ -520,7 +543,7 @@ static void FN(compute_and_store_fluxes)(FN(run) * h, R dt) {
       FN(trace)(h, qLocN, dqN, dqN + 4, dtdx, dtdy, FACE_YMAX, qleft);
       tmp = qleft[2]; qleft[2] = qleft[3]; qleft[3] = tmp;
       tmp = qright[2]; qright[2] = qright[3]; qright[3] = tmp;
-      FN(solve)(h, qleft, qright, qgdnv, flux_y);
+      FN(solve)(h, qleft, qright, qgdnv, flux_y, FN(tape_slot)(h, i, j, 1));
       tmp = flux_y[2]; flux_y[2] = flux_y[3]; flux_y[3] = tmp;
       for (int v = 0; v < NBVAR; ++v) AT(h->fy, i, j, v + 1) = flux_y[v] * dtdy;
     }
@@ -648,6 +671,69 @@ void FN(oracle_kat)(void *hv, int func, int n, const double *x, double *y) {
       default: break;
     }
   }
+}
+
+/* For the adjoint checks: the 4-point central difference
+ *   (8 (f(+e) - f(-e)) - (f(+2e) - f(-2e))) / (12 e)
+ * of the frozen-dt rollout map f: U0 -> U_n (n = nsteps steps of
+ * oracle_godunov with dt = dts[0..n-1]) along dir, evaluated in R: the
+ * perturbed starts U0 + s e dir are formed in R from the double inputs, and
+ * the differences are taken in R, element by element, before the result is
+ * rounded to double.  Buffer u2 starts each evaluation at zero (the
+ * reference's never-written u2).  freeze_newton: the Newton iteration count
+ * of every face and step is recorded in an unperturbed run first and the
+ * perturbed runs repeat it (the map as executed at U0, which the tangent and
+ * adjoint linearise; the stop rule's switches are jumps of the size of the
+ * next Newton correction, which a difference quotient divides by e).
+ * weight == NULL: out (isize*jsize*4, the Fortran layout) receives the
+ * derivative of the final state; else out[0] receives the derivative of
+ * sum(weight * U_n).  The run's arrays are overwritten.  Returns 0, or -1
+ * when a scratch allocation fails. */
+int FN(oracle_fd)(void *hv, const double *u0, const double *dir, int nsteps, const double *dts, double eps,
+                  const double *weight, int freeze_newton, double *out) {
+  FN(run) *h = (FN(run) *)hv;
+  const size_t n = (size_t)h->isize * (size_t)h->jsize * NBVAR;
+  static const int mult[4] = {1, -1, 2, -2};
+  static const int coef[4] = {8, -8, -1, 1};
+  R *acc = (R *)calloc(n, sizeof(R));
+  if (!acc) return -1;
+  if (freeze_newton) {
+    h->tape = (unsigned char *)calloc((size_t)(nsteps > 0 ? nsteps : 1) * (size_t)h->isize * (size_t)h->jsize * 2, 1);
+    if (!h->tape) {
+      free(acc);
+      return -1;
+    }
+  }
+  const R e = (R)eps;
+  for (int m = freeze_newton ? -1 : 0; m < 4; ++m) {
+    /* m = -1: the unperturbed run that records the Newton counts */
+    h->tape_mode = freeze_newton ? (m < 0 ? 1 : 2) : 0;
+    for (size_t k = 0; k < n; ++k) {
+      h->u[k] = m < 0 ? (R)u0[k] : (R)u0[k] + ((R)mult[m] * e) * (R)dir[k];
+      h->u2[k] = (R)0;
+    }
+    for (int s = 0; s < nsteps; ++s) {
+      h->tape_step = s;
+      FN(oracle_godunov)(h, s, dts[s]);
+    }
+    if (m < 0) continue;
+    const R *fin = (nsteps % 2 == 0) ? h->u : h->u2;
+    for (size_t k = 0; k < n; ++k) acc[k] += (R)coef[m] * fin[k];
+  }
+  h->tape_mode = 0;
+  free(h->tape);
+  h->tape = NULL;
+  const R den = (R)12 * e;
+  if (weight) {
+    R sum = (R)0;
+    for (size_t k = 0; k < n; ++k)
+      if (weight[k] != 0.0) sum += (R)weight[k] * (acc[k] / den);
+    out[0] = (double)sum;
+  } else {
+    for (size_t k = 0; k < n; ++k) out[k] = (double)(acc[k] / den);
+  }
+  free(acc);
+  return 0;
 }
 
 #undef AT

@@ -34,7 +34,7 @@ def lib():
             raise FileNotFoundError(f"{ORACLE_SO} missing: run `make -C oracle`")
         L = ctypes.CDLL(str(ORACLE_SO))
         dp = ctypes.POINTER(ctypes.c_double)
-        for s in ("f64", "f32"):
+        for s in ("f64", "f32", "f80", "f128"):
             f = getattr(L, f"oracle_create_{s}")
             f.restype = ctypes.c_void_p
             f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
@@ -57,6 +57,9 @@ def lib():
             getattr(L, f"oracle_get_{s}").argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
             getattr(L, f"oracle_set_{s}").argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
             getattr(L, f"oracle_kat_{s}").argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, dp, dp]
+            f = getattr(L, f"oracle_fd_{s}")
+            f.restype = ctypes.c_int
+            f.argtypes = [ctypes.c_void_p, dp, dp, ctypes.c_int, dp, ctypes.c_double, dp, ctypes.c_int, dp]
         L.oracle_threads.restype = ctypes.c_int
         L.oracle_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -131,6 +134,25 @@ class Oracle:
     def set(self, which: int, arr: np.ndarray):
         a = np.asfortranarray(arr, dtype=np.float64).reshape(-1, order="F").copy()
         self._f("set")(self.h, which, _dp(a))
+
+    def fd(self, u0: np.ndarray, direction: np.ndarray, dts, eps: float, weight: np.ndarray | None = None,
+           freeze_newton: bool = False):
+        """4-point central difference, in this oracle's precision, of the
+        frozen-dt rollout U0 -> U_n (n = len(dts)) along `direction`: the
+        (isize, jsize, 4) derivative of U_n, or with `weight` the derivative
+        of sum(weight * U_n).  freeze_newton: every face keeps the Newton
+        iteration count of the unperturbed run.  u2 starts at zero; the run's
+        arrays are overwritten."""
+        a = np.asfortranarray(u0, dtype=np.float64).reshape(-1, order="F").copy()
+        d = np.asfortranarray(direction, dtype=np.float64).reshape(-1, order="F").copy()
+        t = np.ascontiguousarray(dts, dtype=np.float64)
+        w = None if weight is None else np.asfortranarray(weight, dtype=np.float64).reshape(-1, order="F").copy()
+        out = np.zeros(1 if w is not None else a.size, dtype=np.float64)
+        rc = self._f("fd")(self.h, _dp(a), _dp(d), t.size, _dp(t), eps, None if w is None else _dp(w),
+                           1 if freeze_newton else 0, _dp(out))
+        if rc != 0:
+            raise MemoryError("oracle_fd scratch allocation failed")
+        return float(out[0]) if w is not None else out.reshape((self.isize, self.jsize, 4), order="F")
 
     def kat(self, func: int, x: np.ndarray) -> np.ndarray:
         nin, nout = KAT_SHAPES[func]

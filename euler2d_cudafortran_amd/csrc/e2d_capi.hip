@@ -9,6 +9,7 @@
 #include <cstring>
 #include <new>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "../../include/euler2d_hip.h"
@@ -860,6 +861,93 @@ int e2d_copy_from_buffer(e2d_ctx* c, int buf, void* d_dst) {
   if (!d_dst) return fail(E2D_ERR_ARG, "NULL argument");
   HIP_TRY(hipMemcpyAsync(d_dst, c->b.u[buf], state_bytes(c), hipMemcpyDeviceToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  return E2D_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// the device-resident step loop on caller memory, allocating nothing
+
+namespace {
+
+// Scratch of e2d_run_steps_noalloc: the two running-invDt slots that the
+// step loop ping-pongs between (a context keeps them in its own allocation),
+// 2 x 32 partial maxima on separate 128-byte lines, 8 KB.  Each device holds
+// its own copy from the first call on it until the process ends; every call
+// on a device uses that copy, so calls that overlap on one device race.
+constexpr int kNoallocSlots = 2;
+static __device__ __align__(256) double scratch[kNoallocSlots * e2d::kInvdtSlot];
+
+// Host code never passes `scratch` itself (its name is a host-side stand-in,
+// not a device address): it uses the address hipGetSymbolAddress returns,
+// fetched once per device with that device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_of_current_device(double** out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(E2D_ERR_HIP, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(E2D_ERR_HIP, "hipGetSymbolAddress returned NULL for the scratch array");
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return E2D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int e2d_run_steps_noalloc(const e2d_params* p, void* d_u, void* d_u2, int64_t pitch, int nsteps, int nStep,
+                          double* d_invdt, void* stream) {
+  g_err[0] = '\0';
+  if (int rc = validate(p)) return rc;
+  if (p->implementation_version != E2D_IMPL_FUSED)
+    return fail(E2D_ERR_ARG,
+                "e2d_run_steps_noalloc runs implementationVersion 0 only (versions 1 and 2 keep stage arrays "
+                "in device memory); got %d", p->implementation_version);
+  if (!d_u || !d_u2 || !d_invdt) return fail(E2D_ERR_ARG, "NULL device pointer");
+  if (d_u == d_u2) return fail(E2D_ERR_ARG, "the two state buffers must be distinct");
+  const int64_t isize = p->nx + 4, jsize = p->ny + 4;
+  if (pitch < isize) return fail(E2D_ERR_ARG, "pitch %lld is below isize %lld", (long long)pitch, (long long)isize);
+  if (pitch * jsize > (int64_t(1) << 31) / 4)
+    return fail(E2D_ERR_ARG, "pitch %lld x jsize %lld too large", (long long)pitch, (long long)jsize);
+  if (nsteps < 0 || nStep < 0) return fail(E2D_ERR_ARG, "nsteps and nStep must be >= 0");
+  if (nsteps == 0) return E2D_OK;
+  double* scr = nullptr;
+  if (int rc = scratch_of_current_device(&scr)) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  e2d::DevBuffers b;
+  b.u[0] = d_u;
+  b.u[1] = d_u2;
+  b.pitch = pitch;
+  b.plane = pitch * jsize;
+  b.scal = scr;
+  // slot 0: the running invDt folded with the current input's CFL rate
+  // (compute_dt); slot 1: zero until the first step folds into it
+  HIP_TRY(e2d::launch_fill_from(scr, e2d::kInvdtSlot, d_invdt, st));
+  HIP_TRY(e2d::launch_fill(scr + e2d::kInvdtSlot, e2d::kInvdtSlot, 0.0, st));
+  HIP_TRY(e2d::launch_max_invdt(*p, b, nStep % 2, 0, st));
+  e2d::StepCfg cfg;
+  cfg.dt_from_device = true;
+  cfg.track_dt = true;
+  int k = 0;
+  for (int s = 0, n = nStep; s < nsteps; ++s, ++n) {
+    cfg.buf_in = n % 2;
+    cfg.buf_out = 1 - cfg.buf_in;
+    cfg.slot_cur = k;
+    cfg.slot_next = 1 - k;
+    HIP_TRY(e2d::launch_fused_step(*p, b, cfg, st));
+    k = 1 - k;
+  }
+  HIP_TRY(e2d::launch_slot_value(scr + k * e2d::kInvdtSlot, d_invdt, st));
   return E2D_OK;
 }
 

@@ -56,6 +56,10 @@ hipError_t launch_boundaries(const e2d_params& p, const DevBuffers& b, int buf, 
 // atomic max of the CFL rate over the compute_dt window of buffer buf into invDt slot `slot`
 hipError_t launch_max_invdt(const e2d_params& p, const DevBuffers& b, int buf, int slot, hipStream_t s);
 hipError_t launch_fill(double* dst, int n, double v, hipStream_t s);
+// dst[0..n) = *src (src a device address)
+hipError_t launch_fill_from(double* dst, int n, const double* src, hipStream_t s);
+// *dst = value of the running-invDt slot starting at `slot` (the maximum of its parts)
+hipError_t launch_slot_value(const double* slot, double* dst, hipStream_t s);
 // *flag (device) = 1 when the two device arrays of `bytes` bytes differ in any bit, else 0
 hipError_t launch_words_differ(const void* a, const void* b, size_t bytes, double* flag, hipStream_t s);
 hipError_t launch_init_problem(const e2d_params& p, const DevBuffers& b, int problem,

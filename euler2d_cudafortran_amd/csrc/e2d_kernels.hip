@@ -580,6 +580,27 @@ hipError_t launch_fill(double* dst, int n, double v, hipStream_t s) {
   return hipGetLastError();
 }
 
+__global__ void k_fill_from(double* dst, int n, const double* src) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = *src;
+}
+
+hipError_t launch_fill_from(double* dst, int n, const double* src, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_from, dim3((n + 63) / 64), dim3(64), 0, s, dst, n, src);
+  return hipGetLastError();
+}
+
+// one wavefront: *dst = the value of the running-invDt slot at `slot` (the maximum of its parts)
+__global__ __launch_bounds__(64) void k_slot_value(const double* slot, double* dst) {
+  const double v = invdt_read_wave(slot, 0);
+  if (threadIdx.x == 0) *dst = v;
+}
+
+hipError_t launch_slot_value(const double* slot, double* dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_slot_value, dim3(1), dim3(64), 0, s, slot, dst);
+  return hipGetLastError();
+}
+
 // *flag = 1 when two device arrays of n 64-bit words differ in any bit
 __global__ void k_words_differ(const unsigned long long* a, const unsigned long long* b, int64_t n, double* flag) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;

@@ -92,6 +92,18 @@ module euler2d_hip
        integer(c_int), intent(inout) :: nStep
      end function e2d_run_steps
 
+     !> no counterpart in the reference: the e2d_run_steps loop on caller-owned
+     !! device buffers for callers that cannot allocate device memory
+     integer(c_int) function e2d_run_steps_noalloc(p, d_u, d_u2, pitch, nsteps, nStep, d_invdt, stream) &
+          bind(C, name='e2d_run_steps_noalloc')
+       import :: c_int, c_int64_t, c_ptr, e2d_params_t
+       type(e2d_params_t), intent(in) :: p
+       type(c_ptr), value             :: d_u, d_u2
+       integer(c_int64_t), value      :: pitch
+       integer(c_int), value          :: nsteps, nStep
+       type(c_ptr), value             :: d_invdt, stream
+     end function e2d_run_steps_noalloc
+
      integer(c_int) function e2d_get_dt(ctx, dt) bind(C, name='e2d_get_dt')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value          :: ctx

@@ -96,6 +96,9 @@ _SIGS = {
     "e2d_field_download": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "e2d_copy_to_buffer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     "e2d_copy_from_buffer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "e2d_run_steps_noalloc": (ctypes.c_int, [ctypes.POINTER(_Params), ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                             ctypes.c_void_p]),
 }
 
 
@@ -438,6 +441,17 @@ class HydroRun:
             self.godunov_unsplit(self.nStep, dt)
             self.nStep += 1
         return dts
+
+
+def run_steps_noalloc(params: HydroParams, d_u: int, d_u2: int, pitch: int, nsteps: int, n_step: int,
+                      d_invdt: int, stream: int = 0):
+    """e2d_run_steps_noalloc: nsteps device-resident steps on caller-owned device
+    buffers (addresses d_u, d_u2 in the device layout, row pitch `pitch`), the
+    running invDt in the device double at d_invdt; enqueued on `stream`
+    (0 = the null stream), nothing allocated and nothing waited for."""
+    cp = params.to_c()
+    _check(load_library().e2d_run_steps_noalloc(ctypes.byref(cp), d_u, d_u2, pitch, nsteps, n_step, d_invdt,
+                                                stream))
 
 
 def pointwise(params: HydroParams, func: int, x: np.ndarray) -> np.ndarray:

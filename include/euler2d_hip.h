@@ -165,6 +165,26 @@ int e2d_godunov_unsplit(e2d_ctx *ctx, int nStep, double dt);
  * nsteps times. */
 int e2d_run_steps(e2d_ctx *ctx, int nsteps, int *nStep);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the loop of e2d_run_steps (implementationVersion 0, any
+ * precision and solver) on caller-owned state buffers, without a context.
+ * d_u and d_u2 are buffers 0 and 1 in the device layout of e2d_device_buffer
+ * (element (i,j,v) at [(v*jsize + j)*pitch + i], storage type of the
+ * precision, pitch >= nx+4); nStep selects the input buffer (nStep % 2) as in
+ * e2d_run_steps; *d_invdt (one device double) holds the running invDt on
+ * entry and receives the new running maximum.  Everything is enqueued on
+ * `stream` (a hipStream_t; NULL = the null stream) and nothing waits.
+ * Equivalent, bit for bit, to e2d_run_steps on a context whose buffers and
+ * running invDt hold the same values.  It calls no allocating routine: its
+ * only scratch (the two running-invDt slots, 8 KB) is a static device array
+ * that each device keeps from the first call on it until the process ends,
+ * so calls that overlap on one device race.  Grids up to the e2d_create
+ * limit (pitch * (ny+4) * 4 < 2^31 elements); implementation versions 1 and
+ * 2 are rejected (their stage arrays would need allocations). */
+int e2d_run_steps_noalloc(const e2d_params *p, void *d_u, void *d_u2, int64_t pitch, int nsteps, int nStep,
+                          double *d_invdt, void *stream);
+
 /* dt that the next step will use (cfl / running invDt); synchronous. */
 int e2d_get_dt(e2d_ctx *ctx, double *dt);
 

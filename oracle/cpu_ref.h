@@ -49,7 +49,7 @@ enum {
   void oracle_set_##S(void *h, int which, const double *in);                                  \
   void oracle_kat_##S(void *h, int func, int n, const double *x, double *y);                   \
   int oracle_fd_##S(void *h, const double *u0, const double *dir, int nsteps, const double *dts, \
-                    double eps, const double *weight, int freeze_newton, double *out);
+                    double eps, const double *weight, int freeze, double *out);
 
 ORACLE_DECLARE(f64)
 ORACLE_DECLARE(f32)

@@ -26,15 +26,31 @@ typedef struct {
   R gamma0, gamma6, cfl, smallr, smallc, smallp, smallpp, dx, dy;
   R *u, *u2, *q, *fx, *fy;
   R invDt; /* SAVEd running max of compute_dt (serial_ref/m_run.f90:257) */
-  /* Newton iteration counts of the two-shock solver per step and face, for
-   * oracle_fd only: mode 1 records them, mode 2 replays them (0: unused) */
+  /* decisions of the two-shock solver per step and face (Newton count and
+   * sampling branches, FN(riemann_n)), for oracle_fd only: mode 1 records
+   * them, mode 2 replays them (0: unused) */
   unsigned char *tape;
+  unsigned char *ltape; /* the limiters' decisions per step and cell (FN(slope_at)) */
   int tape_mode, tape_step;
 } FN(run);
 
 #define NBVAR 4
 /* 1-based Fortran indexing, as in the reference: a(i,j,v) */
 #define AT(a, i, j, v) (a)[((size_t)((v)-1) * (size_t)h->jsize + (size_t)((j)-1)) * (size_t)h->isize + (size_t)((i)-1)]
+
+/* tape entry (2 bytes) of face dir (0: x, 1: y) of cell (i, j) in the current step */
+static inline unsigned char *FN(tape_slot)(const FN(run) * h, int i, int j, int dir) {
+  if (!h->tape || !h->tape_mode) return NULL;
+  return h->tape + ((((size_t)h->tape_step * (size_t)h->jsize + (size_t)(j - 1)) * (size_t)h->isize +
+                     (size_t)(i - 1)) * 2 + (size_t)dir) * 2;
+}
+
+/* limiter tape entry (8 bytes: X then Y slopes of the 4 variables) of cell (i, j) in the current step */
+static inline unsigned char *FN(ltape_slot)(const FN(run) * h, int i, int j) {
+  if (!h->ltape || !h->tape_mode) return NULL;
+  return h->ltape + (((size_t)h->tape_step * (size_t)h->jsize + (size_t)(j - 1)) * (size_t)h->isize +
+                     (size_t)(i - 1)) * 8;
+}
 
 static inline R FN(fmax2)(R a, R b) { return a > b ? a : b; }
 static inline R FN(fmin2)(R a, R b) { return a < b ? a : b; }
@@ -85,6 +101,49 @@ static inline void FN(slope)(const FN(run) * h, const R q[4], const R qpx[4], co
     }
   } else {
     for (int v = 0; v < 8; ++v) dq[v] = (R)0;
+  }
+}
+
+/* One limiter of FN(slope) whose decisions oracle_fd records (mode 1: the
+ * reference's arithmetic, and the byte *t) or replays (mode 2: the same
+ * operands combined as *t says).  Bits: 0 dcen >= 0, 1 dlft >= 0, 2 drgt >= 0,
+ * 3 |dlft| < |drgt|, 4 dlft*drgt <= 0, 5 dlim < |dcen|. */
+static inline R FN(limiter_t)(R st, R qm, R q0, R qp, unsigned char *t, int mode) {
+  const R dlft = st * (q0 - qm);
+  const R drgt = st * (qp - q0);
+  const R dcen = (R)0.5 * (qp - qm);
+  if (mode == 1) {
+    const R dsgn = (dcen >= (R)0) ? (R)1 : (R)-1;
+    const R slop = MIN2(ABS(dlft), ABS(drgt));
+    R dlim = slop;
+    if ((dlft * drgt) <= (R)0) dlim = (R)0;
+    *t = (unsigned char)((dcen >= (R)0) | ((dlft >= (R)0) << 1) | ((drgt >= (R)0) << 2) |
+                         ((ABS(dlft) < ABS(drgt)) << 3) | (((dlft * drgt) <= (R)0) << 4) |
+                         ((dlim < ABS(dcen)) << 5));
+    return dsgn * MIN2(dlim, ABS(dcen));
+  }
+  const int b = *t;
+  const R al = (b & 2) ? dlft : -dlft;
+  const R ar = (b & 4) ? drgt : -drgt;
+  const R dlim = (b & 16) ? (R)0 : ((b & 8) ? al : ar);
+  const R m = (b & 32) ? dlim : ((b & 1) ? dcen : -dcen);
+  return (b & 1) ? m : -m;
+}
+
+/* FN(slope) of cell (i, j) with its limiter decisions taped (oracle_fd) */
+static inline void FN(slope_at)(const FN(run) * h, int i, int j, const R q[4], const R qpx[4], const R qmx[4],
+                                const R qpy[4], const R qmy[4], R dq[8]) {
+  unsigned char *t = FN(ltape_slot)(h, i, j);
+  if (!t || !(h->slope_type == 1 || h->slope_type == 2)) {
+    FN(slope)(h, q, qpx, qmx, qpy, qmy, dq);
+    return;
+  }
+  /* a cell's slopes are formed again as a neighbour's, possibly on another
+   * thread: every recording of a cell writes the same bytes */
+  const R st = (R)h->slope_type;
+  for (int v = 0; v < NBVAR; ++v) {
+    dq[v] = FN(limiter_t)(st, qmx[v], q[v], qpx[v], t + v, h->tape_mode);
+    dq[4 + v] = FN(limiter_t)(st, qmy[v], q[v], qpy[v], t + 4 + v, h->tape_mode);
   }
 }
 
@@ -140,11 +199,31 @@ static inline void FN(cmpflx)(const FN(run) * h, const R g[4], R f[4]) {
   f[1] = g[2] * (etot + g[1]);
 }
 
+/* max(a, b) and |x| whose outcome oracle_fd records (bit `bit` of rec[1])
+ * or replays (from rep[1]); with neither, MAX2 and ABS themselves */
+static inline R FN(max_t)(R a, R b, const unsigned char *rep, unsigned char *rec, int bit) {
+  if (rep) return ((rep[1] >> bit) & 1) ? a : b;
+  if (rec && a > b) rec[1] = (unsigned char)(rec[1] | (1 << bit));
+  return MAX2(a, b);
+}
+static inline R FN(abs_t)(R x, const unsigned char *rep, unsigned char *rec, int bit) {
+  if (rep) return ((rep[1] >> bit) & 1) ? x : -x;
+  if (rec && x >= (R)0) rec[1] = (unsigned char)(rec[1] | (1 << bit));
+  return ABS(x);
+}
+
 /* riemann_2d: serial_ref/m_utils.f90:349-492.  Returns the Newton iteration
  * count (not part of the reference interface; used to grade test coverage).
- * fixed < 0 keeps the reference's stop rule; fixed >= 0 runs exactly that
- * many iterations instead (oracle_fd's frozen counts). */
-static inline int FN(riemann_n)(const FN(run) * h, const R ql[4], const R qr[4], R g[4], R f[4], int fixed) {
+ * For oracle_fd's frozen decisions: `record` (if not NULL) receives two
+ * bytes, the first holding the Newton count (bits 0-3) and the outcomes of
+ * the four branch tests, u* >= 0 (bit 4), p* >= po (bit 5), spout < 0
+ * (bit 6), spin > 0 (bit 7), the second the sides taken by the maxima and
+ * the absolute value the solver passes through (first guess 0, c_o and c*
+ * floors, r* floor, the sampling's denominator and |spout + spin|);
+ * `replay` (if not NULL) imposes two such bytes instead of evaluating the
+ * stop rule and the tests.  Both NULL: the reference's solver. */
+static inline int FN(riemann_n)(const FN(run) * h, const R ql[4], const R qr[4], R g[4], R f[4],
+                                const unsigned char *replay, unsigned char *record) {
   const R rl = MAX2(ql[0], h->smallr);
   const R ul = ql[2];
   const R pl = MAX2(ql[1], rl * h->smallp);
@@ -154,15 +233,16 @@ static inline int FN(riemann_n)(const FN(run) * h, const R ql[4], const R qr[4],
 
   const R cl = h->gamma0 * pl * rl;
   const R cr = h->gamma0 * pr * rr;
+  if (record) record[1] = 0;
 
   R wl = SQRT(cl);
   R wr = SQRT(cr);
-  R pstar = MAX2(((wr * pl + wl * pr) + wl * wr * (ul - ur)) / (wl + wr), (R)0);
+  R pstar = FN(max_t)(((wr * pl + wl * pr) + wl * wr * (ul - ur)) / (wl + wr), (R)0, replay, record, 0);
   R pold = pstar;
   R conv = (R)1;
 
   int iter = 0;
-  while (fixed >= 0 ? iter < fixed : (iter < h->niter_riemann && conv > LIT_F32(1e-6))) {
+  while (replay ? iter < (*replay & 15) : (iter < h->niter_riemann && conv > LIT_F32(1e-6))) {
     R wwl = SQRT(cl * ((R)1 + h->gamma6 * (pold - pl) / pl));
     R wwr = SQRT(cr * ((R)1 + h->gamma6 * (pold - pr) / pr));
     R qlw = (R)2.0 * wwl * wwl * wwl / (wwl * wwl + cl);
@@ -180,7 +260,8 @@ static inline int FN(riemann_n)(const FN(run) * h, const R ql[4], const R qr[4],
   wr = SQRT(cr * ((R)1 + h->gamma6 * (pstar - pr) / pr));
 
   const R ustar = (R)0.5 * (ul + (pl - pstar) / wl + ur - (pr - pstar) / wr);
-  const R sgnm = (ustar >= (R)0) ? (R)1.0 : (R)-1.0;
+  const int b_sg = replay ? (*replay >> 4) & 1 : (ustar >= (R)0);
+  const R sgnm = b_sg ? (R)1.0 : (R)-1.0;
 
   R ro, uo, po, wo;
   if (sgnm > (R)0) {
@@ -188,37 +269,41 @@ static inline int FN(riemann_n)(const FN(run) * h, const R ql[4], const R qr[4],
   } else {
     ro = rr; uo = ur; po = pr; wo = wr;
   }
-  const R co = MAX2(h->smallc, SQRT(ABS(h->gamma0 * po / ro)));
-  const R rstar = MAX2(ro / ((R)1 + ro * (po - pstar) / (wo * wo)), h->smallr);
-  const R cstar = MAX2(h->smallc, SQRT(ABS(h->gamma0 * pstar / rstar)));
+  const R co = FN(max_t)(h->smallc, SQRT(ABS(h->gamma0 * po / ro)), replay, record, 1);
+  const R rstar = FN(max_t)(ro / ((R)1 + ro * (po - pstar) / (wo * wo)), h->smallr, replay, record, 2);
+  const R cstar = FN(max_t)(h->smallc, SQRT(ABS(h->gamma0 * pstar / rstar)), replay, record, 3);
 
   R spout = co - sgnm * uo;
   R spin = cstar - sgnm * ustar;
   const R ushock = wo / ro - sgnm * uo;
-  if (pstar >= po) {
+  const int b_shock = replay ? (*replay >> 5) & 1 : (pstar >= po);
+  if (b_shock) {
     spin = ushock;
     spout = ushock;
   }
-  const R scr = MAX2(spout - spin, h->smallc + ABS(spout + spin));
+  const R scr = FN(max_t)(spout - spin, h->smallc + FN(abs_t)(spout + spin, replay, record, 5), replay, record, 4);
   const R frac = (R)0.5 * ((R)1 + (spout + spin) / scr);
 
   g[0] = frac * rstar + ((R)1 - frac) * ro;
   g[2] = frac * ustar + ((R)1 - frac) * uo;
   g[1] = frac * pstar + ((R)1 - frac) * po;
-  if (spout < (R)0) {
+  const int b_out = replay ? (*replay >> 6) & 1 : (spout < (R)0);
+  const int b_in = replay ? (*replay >> 7) & 1 : (spin > (R)0);
+  if (b_out) {
     g[0] = ro; g[2] = uo; g[1] = po;
   }
-  if (spin > (R)0) {
+  if (b_in) {
     g[0] = rstar; g[2] = ustar; g[1] = pstar;
   }
   g[3] = (sgnm > (R)0) ? ql[3] : qr[3];
 
   FN(cmpflx)(h, g, f);
+  if (record) record[0] = (unsigned char)(iter | (b_sg << 4) | (b_shock << 5) | (b_out << 6) | (b_in << 7));
   return iter;
 }
 
 static inline int FN(riemann)(const FN(run) * h, const R ql[4], const R qr[4], R g[4], R f[4]) {
-  return FN(riemann_n)(h, ql, qr, g, f, -1);
+  return FN(riemann_n)(h, ql, qr, g, f, NULL, NULL);
 }
 
 /* riemann_2d_hllc: serial_ref/m_utils.f90:534-644 */
@@ -277,25 +362,16 @@ static inline void FN(hllc)(const FN(run) * h, const R ql[4], const R qr[4], R g
   f[3] = (f[0] > (R)0) ? f[0] * ql[3] : f[0] * qr[3];
 }
 
-/* slot: this face's entry of the Newton-count tape, or NULL */
+/* slot: this face's entry of the decision tape, or NULL */
 static inline void FN(solve)(const FN(run) * h, const R ql[4], const R qr[4], R g[4], R f[4],
                              unsigned char *slot) {
-  if (h->riemann_solver == 2) {
+  if (h->riemann_solver == 2)
     FN(hllc)(h, ql, qr, g, f);
-  } else if (slot && h->tape_mode == 2) {
-    (void)FN(riemann_n)(h, ql, qr, g, f, *slot);
-  } else {
-    const int k = FN(riemann)(h, ql, qr, g, f);
-    if (slot && h->tape_mode == 1) *slot = (unsigned char)k;
-  }
+  else
+    (void)FN(riemann_n)(h, ql, qr, g, f, slot && h->tape_mode == 2 ? slot : NULL,
+                        slot && h->tape_mode == 1 ? slot : NULL);
 }
 
-/* tape entry of face dir (0: x, 1: y) of cell (i, j) in the current step */
-static inline unsigned char *FN(tape_slot)(const FN(run) * h, int i, int j, int dir) {
-  if (!h->tape || !h->tape_mode) return NULL;
-  return h->tape + ((((size_t)h->tape_step * (size_t)h->jsize + (size_t)(j - 1)) * (size_t)h->isize +
-                     (size_t)(i - 1)) * 2 + (size_t)dir);
-}
 
 /* ------------------------------------------------------------------------ */
 
@@ -518,14 +594,14 @@ static void FN(compute_and_store_fluxes)(FN(run) * h, R dt) {
       FN(load_q)(h, i - 1, j, qN1);
       FN(load_q)(h, i, j + 1, qN2);
       FN(load_q)(h, i, j - 1, qN3);
-      FN(slope)(h, qLoc, qN0, qN1, qN2, qN3, dq);
+      FN(slope_at)(h, i, j, qLoc, qN0, qN1, qN2, qN3, dq);
 
       FN(load_q)(h, i - 1, j, qLocN);
       FN(load_q)(h, i, j, qN0);
       FN(load_q)(h, i - 2, j, qN1);
       FN(load_q)(h, i - 1, j + 1, qN2);
       FN(load_q)(h, i - 1, j - 1, qN3);
-      FN(slope)(h, qLocN, qN0, qN1, qN2, qN3, dqN);
+      FN(slope_at)(h, i - 1, j, qLocN, qN0, qN1, qN2, qN3, dqN);
 
       FN(trace)(h, qLoc, dq, dq + 4, dtdx, dtdy, FACE_XMIN, qright);
       FN(trace)(h, qLocN, dqN, dqN + 4, dtdx, dtdy, FACE_XMAX, qleft);
@@ -537,7 +613,7 @@ static void FN(compute_and_store_fluxes)(FN(run) * h, R dt) {
       FN(load_q)(h, i - 1, j - 1, qN1);
       FN(load_q)(h, i, j, qN2);
       FN(load_q)(h, i, j - 2, qN3);
-      FN(slope)(h, qLocN, qN0, qN1, qN2, qN3, dqN);
+      FN(slope_at)(h, i, j - 1, qLocN, qN0, qN1, qN2, qN3, dqN);
 
       FN(trace)(h, qLoc, dq, dq + 4, dtdx, dtdy, FACE_YMIN, qright);
       FN(trace)(h, qLocN, dqN, dqN + 4, dtdx, dtdy, FACE_YMAX, qleft);
@@ -680,34 +756,47 @@ void FN(oracle_kat)(void *hv, int func, int n, const double *x, double *y) {
  * perturbed starts U0 + s e dir are formed in R from the double inputs, and
  * the differences are taken in R, element by element, before the result is
  * rounded to double.  Buffer u2 starts each evaluation at zero (the
- * reference's never-written u2).  freeze_newton: the Newton iteration count
- * of every face and step is recorded in an unperturbed run first and the
- * perturbed runs repeat it (the map as executed at U0, which the tangent and
- * adjoint linearise; the stop rule's switches are jumps of the size of the
- * next Newton correction, which a difference quotient divides by e).
+ * reference's never-written u2).  freeze: the decisions of every face, cell
+ * and step are recorded in an unperturbed run first and the perturbed runs
+ * repeat them: the two-shock solver's (Newton count, upwind side, shock or
+ * rarefaction, the two sampling tests, its maxima and |spout + spin|) and the
+ * limiters' (signs, which of |dl|, |dr|, |dc| is taken, the extremum test).
+ * That is the map as executed at U0, smooth in U0, which the tangent and
+ * adjoint linearise.  Free, the solver jumps where its decisions switch (the
+ * stop rule by the size of the next Newton correction, the sonic-point blend
+ * by order smallc / |spin|, the shock test in spin), which a difference
+ * quotient divides by e, and every limiter kink within 2 e leaves an O(1)
+ * error in that cell's slope derivative.  The floors against smallr, smallp
+ * and the Newton step's -p stay free: they act only near vacuum.
  * weight == NULL: out (isize*jsize*4, the Fortran layout) receives the
  * derivative of the final state; else out[0] receives the derivative of
  * sum(weight * U_n).  The run's arrays are overwritten.  Returns 0, or -1
  * when a scratch allocation fails. */
 int FN(oracle_fd)(void *hv, const double *u0, const double *dir, int nsteps, const double *dts, double eps,
-                  const double *weight, int freeze_newton, double *out) {
+                  const double *weight, int freeze, double *out) {
   FN(run) *h = (FN(run) *)hv;
   const size_t n = (size_t)h->isize * (size_t)h->jsize * NBVAR;
   static const int mult[4] = {1, -1, 2, -2};
   static const int coef[4] = {8, -8, -1, 1};
   R *acc = (R *)calloc(n, sizeof(R));
   if (!acc) return -1;
-  if (freeze_newton) {
-    h->tape = (unsigned char *)calloc((size_t)(nsteps > 0 ? nsteps : 1) * (size_t)h->isize * (size_t)h->jsize * 2, 1);
-    if (!h->tape) {
+  if (freeze && h->niter_riemann > 15) freeze = 0; /* counts above 15 do not fit the tape */
+  if (freeze) {
+    const size_t cells = (size_t)(nsteps > 0 ? nsteps : 1) * (size_t)h->isize * (size_t)h->jsize;
+    h->tape = (unsigned char *)calloc(cells * 4, 1);
+    h->ltape = (unsigned char *)calloc(cells * 8, 1);
+    if (!h->tape || !h->ltape) {
+      free(h->tape);
+      free(h->ltape);
+      h->tape = h->ltape = NULL;
       free(acc);
       return -1;
     }
   }
   const R e = (R)eps;
-  for (int m = freeze_newton ? -1 : 0; m < 4; ++m) {
-    /* m = -1: the unperturbed run that records the Newton counts */
-    h->tape_mode = freeze_newton ? (m < 0 ? 1 : 2) : 0;
+  for (int m = freeze ? -1 : 0; m < 4; ++m) {
+    /* m = -1: the unperturbed run that records the decisions */
+    h->tape_mode = freeze ? (m < 0 ? 1 : 2) : 0;
     for (size_t k = 0; k < n; ++k) {
       h->u[k] = m < 0 ? (R)u0[k] : (R)u0[k] + ((R)mult[m] * e) * (R)dir[k];
       h->u2[k] = (R)0;
@@ -722,7 +811,8 @@ int FN(oracle_fd)(void *hv, const double *u0, const double *dir, int nsteps, con
   }
   h->tape_mode = 0;
   free(h->tape);
-  h->tape = NULL;
+  free(h->ltape);
+  h->tape = h->ltape = NULL;
   const R den = (R)12 * e;
   if (weight) {
     R sum = (R)0;

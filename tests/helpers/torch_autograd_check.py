@@ -11,8 +11,8 @@ bit, and match the directional derivative of J along a smooth (rho, p)
 direction from a smooth initial state with flow, dJ = sum 2 (rho_N -
 rho_target) d rho_N, with d rho_N the 4-point finite difference of the rollout
 evaluated by the oracle in binary128 at eps = 1e-10 with the frozen dt
-sequence and frozen Newton counts (no truncation or roundoff that a double
-shows; bar 1e-10).  Prints one line "AUTOGRAD_CHECK ok rel=<relative error>"
+sequence and every decision of the step frozen at the unperturbed run's (no
+truncation or roundoff that a double shows; bar 1e-10).  Prints one line "AUTOGRAD_CHECK ok rel=<relative error>"
 on success."""
 import pathlib
 import sys
@@ -101,7 +101,7 @@ def main(niter=10, eps=1e-10, tol=1e-10):
 
     bc = (3, 3, 3, 3)
     o = Oracle(nx, nx, bc=bc, niter_riemann=niter, precision="f128")
-    fd = o.fd(to_host(u0), to_host(du), dts, eps, weight=to_host(weight), freeze_newton=True)
+    fd = o.fd(to_host(u0), to_host(du), dts, eps, weight=to_host(weight), freeze=True)
     rel = abs(fd - ad) / abs(fd)
     print(f"autograd gradient: adjoint {ad:.15e} binary128 finite difference {fd:.15e} rel {rel:.2e}")
     assert rel <= tol, rel

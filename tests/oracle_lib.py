@@ -136,12 +136,13 @@ class Oracle:
         self._f("set")(self.h, which, _dp(a))
 
     def fd(self, u0: np.ndarray, direction: np.ndarray, dts, eps: float, weight: np.ndarray | None = None,
-           freeze_newton: bool = False):
+           freeze: bool = False):
         """4-point central difference, in this oracle's precision, of the
         frozen-dt rollout U0 -> U_n (n = len(dts)) along `direction`: the
         (isize, jsize, 4) derivative of U_n, or with `weight` the derivative
-        of sum(weight * U_n).  freeze_newton: every face keeps the Newton
-        iteration count of the unperturbed run.  u2 starts at zero; the run's
+        of sum(weight * U_n).  freeze: every face keeps the two-shock
+        solver's decisions of the unperturbed run (Newton count, upwind side,
+        shock or rarefaction, sampling).  u2 starts at zero; the run's
         arrays are overwritten."""
         a = np.asfortranarray(u0, dtype=np.float64).reshape(-1, order="F").copy()
         d = np.asfortranarray(direction, dtype=np.float64).reshape(-1, order="F").copy()
@@ -149,7 +150,7 @@ class Oracle:
         w = None if weight is None else np.asfortranarray(weight, dtype=np.float64).reshape(-1, order="F").copy()
         out = np.zeros(1 if w is not None else a.size, dtype=np.float64)
         rc = self._f("fd")(self.h, _dp(a), _dp(d), t.size, _dp(t), eps, None if w is None else _dp(w),
-                           1 if freeze_newton else 0, _dp(out))
+                           1 if freeze else 0, _dp(out))
         if rc != 0:
             raise MemoryError("oracle_fd scratch allocation failed")
         return float(out[0]) if w is not None else out.reshape((self.isize, self.jsize, 4), order="F")

@@ -118,12 +118,12 @@ def test_fd_precisions_and_convergence(slope_type):
     mask = _interior(nx, nx)
     d = np.where(mask, _smooth_state(nx, nx, 21 + slope_type) - 1.0, 0.0)
     o = {prec: Oracle(nx, bc=(3, 3, 3, 3), slope_type=slope_type, precision=prec) for prec in ("f64", "f80", "f128")}
-    same_eps = {prec: o[prec].fd(U0, d, [dt], 1e-6, freeze_newton=True)[mask] for prec in o}
+    same_eps = {prec: o[prec].fd(U0, d, [dt], 1e-6, freeze=True)[mask] for prec in o}
     ref = same_eps["f128"]
     nrm = np.linalg.norm(ref)
     e64 = np.linalg.norm(same_eps["f64"] - ref) / nrm
     e80 = np.linalg.norm(same_eps["f80"] - ref) / nrm
-    conv = {eps: o["f128"].fd(U0, d, [dt], eps, freeze_newton=True)[mask] for eps in (1e-8, 1e-10, 1e-12)}
+    conv = {eps: o["f128"].fd(U0, d, [dt], eps, freeze=True)[mask] for eps in (1e-8, 1e-10, 1e-12)}
     spread = max(np.linalg.norm(conv[e] - conv[1e-10]) for e in conv) / np.linalg.norm(conv[1e-10])
     print(f"slope_type {slope_type}: eps 1e-6: |f64 - f128| {e64:.1e}, |f80 - f128| {e80:.1e}; "
           f"f128 at eps 1e-8..1e-12: spread {spread:.1e}")

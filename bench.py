@@ -426,7 +426,7 @@ def main(argv=None):
             traffic = tr[key]["bytes_per_launch"]
             if "valu_issue_us" in tr[key]:
                 # VALU issue time per launch from the counter run, against this run's kernel time
-                valu = {"bound": "fp64 valu issue", "issue_us": tr[key]["valu_issue_us"],
+                valu = {"bound": tr[key].get("valu_bound", "valu issue"), "issue_us": tr[key]["valu_issue_us"],
                         "kernel_us": round(kernel_ms * 1e3, 1),
                         "frac": round(tr[key]["valu_issue_us"] / (kernel_ms * 1e3), 3),
                         "lane_instructions_per_cell": tr[key]["valu_lane_instructions_per_cell"],
@@ -485,8 +485,14 @@ def main(argv=None):
                    "implementation_version": args.impl,
                    "mcell_updates_per_s_reference_formula": round((nx + 4) ** 2 * args.steps / wall / 1e6, 2),
                    "dt_after_run": dt_final},
+        # "bound" names the roofline that achieved / peak / frac are stated
+        # against (HBM bytes, the stencil's natural roof); "binding" names the
+        # limit that holds in fact, with its issue fraction (VALU issue time
+        # from the counter run over this run's kernel time) when one is known
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "binding": valu["bound"] if valu else "hbm",
+                     "valu_issue_frac": valu["frac"] if valu else None,
                      "limiter": valu,
                      "kernel": {0: "k_step_fused", 1: "materialised step (5 kernels)",
                                 2: "flux-store step (3 kernels)"}[args.impl],

@@ -257,7 +257,10 @@ __global__ __launch_bounds__(256) void k_adj_cells(const AdjArgs<P> a) {
 #ifndef E2D_BAND_GATHER
 #define E2D_BAND_GATHER 1
 #endif
-constexpr int kAdjWaves = 4;      // wavefronts per workgroup
+#ifndef E2D_ADJ_WAVES_PER_BLOCK
+#define E2D_ADJ_WAVES_PER_BLOCK 4
+#endif
+constexpr int kAdjWaves = E2D_ADJ_WAVES_PER_BLOCK;  // wavefronts per workgroup
 constexpr int kAdjStripOut = 60;  // output columns per wavefront
 
 template <class P>

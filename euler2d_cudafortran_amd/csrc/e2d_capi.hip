@@ -430,6 +430,7 @@ int e2d_download_stage(e2d_ctx* c, int stage, double* h) {
 
 int e2d_compute_dt(e2d_ctx* c, int useU, double* dt) {
   if (int rc = check_ctx(c)) return rc;
+  const e2d::TraceRange range("compute_dt");
   if (!dt) return fail(E2D_ERR_ARG, "dt is NULL");
   if (int rc = pull_invdt(c)) return rc;
   const int buf = (useU == 0) ? 0 : 1;
@@ -450,6 +451,7 @@ int e2d_make_boundaries(e2d_ctx* c, int buf) {
 int e2d_godunov_unsplit(e2d_ctx* c, int nStep, double dt) {
   if (int rc = check_ctx(c)) return rc;
   if (nStep < 0) return fail(E2D_ERR_ARG, "nStep must be >= 0");
+  const e2d::TraceRange range("godunov_compute");
   if (int rc = pull_invdt(c)) return rc;
   e2d::StepCfg cfg;
   cfg.buf_in = nStep % 2;
@@ -494,19 +496,23 @@ int e2d_run_steps(e2d_ctx* c, int nsteps, int* nStep) {
       cfg.slot_cur = k;
       cfg.slot_next = 1 - k;
       cfg.track_dt = true;
+      const e2d::TraceRange range("godunov_compute");
       HIP_TRY(timed(c, E2D_TIMER_GODUNOV, [&] { return e2d::launch_fused_step(c->p, c->b, cfg, c->stream); }));
       k = 1 - k;
     }
   } else {
+    // one slot: each step reads dt from slot k in its flux kernels, and its
+    // update kernel then folds the new state's CFL rates into slot k
     for (int s = 0; s < nsteps; ++s, ++n) {
       cfg.buf_in = n % 2;
       cfg.buf_out = 1 - cfg.buf_in;
       cfg.slot_cur = k;
+      cfg.track_dt = true;
+      const e2d::TraceRange range("godunov_compute");
       HIP_TRY(timed(c, E2D_TIMER_BOUNDARIES,
                     [&] { return e2d::launch_boundaries(c->p, c->b, cfg.buf_in, c->stream); }));
       HIP_TRY(timed(c, E2D_TIMER_GODUNOV,
                     [&] { return e2d::launch_materialised_step(c->p, c->b, cfg, c->stream); }));
-      HIP_TRY(timed(c, E2D_TIMER_DT, [&] { return e2d::launch_max_invdt(c->p, c->b, cfg.buf_out, k, c->stream); }));
     }
     c->stages_valid = true;
   }

@@ -2,12 +2,25 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cstdint>
 
 #include "../../include/euler2d_hip.h"
 
 namespace e2d {
+
+// Host-side trace range (roctx; rocprofv3 --marker-trace records it), named
+// after the reference's NVTX ranges (serial_ref/m_run.f90:195,264,336,387,502,571:
+// godunov_compute, compute_dt, convertToPrimitives, computeAndStoreFluxes,
+// updateHydro, make_boundaries).  It spans the enqueue of the kernels, which
+// the trace attributes to it.
+struct TraceRange {
+  explicit TraceRange(const char* name) { roctxRangePushA(name); }
+  ~TraceRange() { roctxRangePop(); }
+  TraceRange(const TraceRange&) = delete;
+  TraceRange& operator=(const TraceRange&) = delete;
+};
 
 constexpr int kNumStages = 9;  // E2D_STAGE_Q .. E2D_STAGE_QYP
 
@@ -41,7 +54,8 @@ struct StepCfg {
   bool dt_from_device = false;  // dt = cfl / (invDt slot slot_cur); else dt_host
   double dt_host = 0.0;
   int slot_cur = 0;
-  bool track_dt = false;  // fold the new state's CFL rate into invDt slot slot_next
+  bool track_dt = false;  // fold the new state's CFL rate into invDt slot slot_next (fused
+                          // step) or slot_cur (materialised steps, after their flux kernels)
   int slot_next = 1;
 };
 

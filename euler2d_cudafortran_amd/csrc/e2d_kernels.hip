@@ -35,10 +35,15 @@ namespace e2d {
 // fused step
 
 constexpr int kStripOut = 60;  // output columns per wavefront
+// Independent wavefronts per workgroup of the fused and v2 flux kernels (their
+// wavefronts share nothing).  One per workgroup: measured on MI355X, the FP64
+// step takes 196.0-197.2 us at 2048^2 against 197.9-198.6 with four, and
+// 680.6-683.4 against 689.5-692.5 at 4096^2, with the same results; the float
+// kernels and the v2 flux kernel time the same either way, eight is slower.
 #ifndef E2D_WAVES_PER_BLOCK
-#define E2D_WAVES_PER_BLOCK 4
+#define E2D_WAVES_PER_BLOCK 1
 #endif
-constexpr int kWavesPerBlock = E2D_WAVES_PER_BLOCK;  // independent wavefronts per workgroup
+constexpr int kWavesPerBlock = E2D_WAVES_PER_BLOCK;
 
 // optional register cap for the fused kernel (minimum resident waves per SIMD)
 #if defined(E2D_FUSED_MIN_WAVES) && E2D_FUSED_MIN_WAVES > 0
@@ -520,6 +525,7 @@ static hipError_t boundaries(const e2d_params& p, const DevBuffers& b, int buf, 
 }
 
 hipError_t launch_boundaries(const e2d_params& p, const DevBuffers& b, int buf, hipStream_t s) {
+  const TraceRange range("make_boundaries");
   return with_precision(p.precision, [&](auto pol) { return boundaries<decltype(pol)>(p, b, buf, s); });
 }
 
@@ -632,10 +638,11 @@ struct MatArgs {
   typename P::S* uout;
   typename P::T *q, *dqx, *dqy, *qxm, *qxp, *qym, *qyp;
   typename P::U *fx, *fy;
-  const double* scal;
+  double* scal;
   int dt_from_device, slot_cur;
   double dt_host;
   int nsegs, seg_rows, seg_groups;  // k_flux_store row segments
+  int track_dt;  // k_update folds the new state's CFL rates into invDt slot slot_cur
 };
 
 // kWave: called with all 64 lanes of the wavefront active (wave-parallel read)
@@ -831,19 +838,41 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_flux_store(const MatArg
   }
 }
 
-// updateHydro over the interior (serial_ref/m_run.f90:504-513)
+// updateHydro over the interior (serial_ref/m_run.f90:504-513).  With
+// track_dt, the CFL rate of each new cell in the compute_dt window [2, n]
+// (0-based) is folded into the running invDt (compute_dt of the new state,
+// serial_ref/m_run.f90:248-312): a wave maximum, a block maximum through LDS,
+// one atomic maximum per block.  This step's dt was read from that slot by the
+// flux kernels, which have completed; the maximum is order-independent.
 template <class P>
 __global__ __launch_bounds__(256) void k_update(const MatArgs<P> a) {
   using U = typename P::U;
   using S = typename P::S;
+  __shared__ double red[4];
   const int i = 2 + blockIdx.x * 64 + (threadIdx.x & 63);
   const int j = 2 + blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (i > a.g.nx + 1 || j > a.g.ny + 1) return;
+  const bool cell = i <= a.g.nx + 1 && j <= a.g.ny + 1;
+  U rate = U(0);
+  if (cell) {
+    U un[4];
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    U flux_tot = a.fx[at(a.g, i, j, v)] - a.fx[at(a.g, i + 1, j, v)];
-    flux_tot = a.fy[at(a.g, i, j, v)] - a.fy[at(a.g, i, j + 1, v)] + flux_tot;
-    a.uout[at(a.g, i, j, v)] = static_cast<S>(static_cast<U>(a.uin[at(a.g, i, j, v)]) + flux_tot);
+    for (int v = 0; v < 4; ++v) {
+      U flux_tot = a.fx[at(a.g, i, j, v)] - a.fx[at(a.g, i + 1, j, v)];
+      flux_tot = a.fy[at(a.g, i, j, v)] - a.fy[at(a.g, i, j + 1, v)] + flux_tot;
+      const S sn = static_cast<S>(static_cast<U>(a.uin[at(a.g, i, j, v)]) + flux_tot);
+      a.uout[at(a.g, i, j, v)] = sn;
+      un[v] = static_cast<U>(sn);
+    }
+    if (a.track_dt && i <= a.g.nx && j <= a.g.ny) rate = cfl_rate<U, typename P::MU>(a.hU, un);
+  }
+  if (!a.track_dt) return;  // uniform: the whole block leaves before the barrier
+  const double m = wave_max(static_cast<double>(rate));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = red[0];
+    for (int k = 1; k < 4; ++k) x = x > red[k] ? x : red[k];
+    invdt_fold(a.scal, a.slot_cur, static_cast<int>(blockIdx.y * gridDim.x + blockIdx.x), x);
   }
 }
 
@@ -871,41 +900,51 @@ static hipError_t materialised_step(const e2d_params& p, const DevBuffers& b, co
   a.dt_from_device = c.dt_from_device ? 1 : 0;
   a.slot_cur = c.slot_cur;
   a.dt_host = c.dt_host;
+  a.track_dt = c.track_dt ? 1 : 0;
   const int S_ = p.nx + 4, J_ = p.ny + 4;
   const dim3 fgrid((S_ - 3 + 63) / 64, (J_ - 3 + 3) / 4);
-  hipLaunchKernelGGL(k_prim<P>, dim3((S_ + 63) / 64, (J_ + 3) / 4), dim3(256), 0, st, a);
-  if (p.implementation_version == E2D_IMPL_FLUX_STORE) {
-    // stored rows and columns [2, size-2]; segment length as for the fused step
-    const bool hllc = p.riemann_solver == E2D_RIEMANN_HLLC;
-    const bool minmod = !hllc && p.slope_type == 1;
-    const void* kernel = hllc ? reinterpret_cast<const void*>(k_flux_store<P, 2>)
-                              : (minmod ? reinterpret_cast<const void*>(k_flux_store<P, 0, true>)
-                                        : reinterpret_cast<const void*>(k_flux_store<P, 0>));
-    thread_local const void* cached_kernel = nullptr;
-    thread_local int cached_wps = 3;
-    if (kernel != cached_kernel) {
-      cached_wps = waves_per_simd(kernel);
-      cached_kernel = kernel;
-    }
-    a.seg_rows = fused_seg_rows(p.ny, cached_wps);
-    a.nsegs = (J_ - 3 + a.seg_rows - 1) / a.seg_rows;
-    a.seg_groups = (a.nsegs + kWavesPerBlock - 1) / kWavesPerBlock;
-    const dim3 mgrid(((S_ - 3 + kStripOut - 1) / kStripOut) * a.seg_groups);
-    if (hllc)
-      hipLaunchKernelGGL((k_flux_store<P, 2>), mgrid, dim3(64 * kWavesPerBlock), 0, st, a);
-    else if (minmod)
-      hipLaunchKernelGGL((k_flux_store<P, 0, true>), mgrid, dim3(64 * kWavesPerBlock), 0, st, a);
-    else
-      hipLaunchKernelGGL((k_flux_store<P, 0>), mgrid, dim3(64 * kWavesPerBlock), 0, st, a);
-  } else {
-    hipLaunchKernelGGL(k_slopes<P>, dim3((S_ - 2 + 63) / 64, (J_ - 2 + 3) / 4), dim3(256), 0, st, a);
-    hipLaunchKernelGGL(k_trace<P>, dim3((S_ - 2 + 63) / 64, (J_ - 2 + 3) / 4), dim3(256), 0, st, a);
-    if (p.riemann_solver == E2D_RIEMANN_HLLC)
-      hipLaunchKernelGGL((k_flux<P, 2>), fgrid, dim3(256), 0, st, a);
-    else
-      hipLaunchKernelGGL((k_flux<P, 0>), fgrid, dim3(256), 0, st, a);
+  {
+    const TraceRange range("convertToPrimitives");
+    hipLaunchKernelGGL(k_prim<P>, dim3((S_ + 63) / 64, (J_ + 3) / 4), dim3(256), 0, st, a);
   }
-  hipLaunchKernelGGL(k_update<P>, dim3((p.nx + 63) / 64, (p.ny + 3) / 4), dim3(256), 0, st, a);
+  {
+    const TraceRange range("computeAndStoreFluxes");
+    if (p.implementation_version == E2D_IMPL_FLUX_STORE) {
+      // stored rows and columns [2, size-2]; segment length as for the fused step
+      const bool hllc = p.riemann_solver == E2D_RIEMANN_HLLC;
+      const bool minmod = !hllc && p.slope_type == 1;
+      const void* kernel = hllc ? reinterpret_cast<const void*>(k_flux_store<P, 2>)
+                                : (minmod ? reinterpret_cast<const void*>(k_flux_store<P, 0, true>)
+                                          : reinterpret_cast<const void*>(k_flux_store<P, 0>));
+      thread_local const void* cached_kernel = nullptr;
+      thread_local int cached_wps = 3;
+      if (kernel != cached_kernel) {
+        cached_wps = waves_per_simd(kernel);
+        cached_kernel = kernel;
+      }
+      a.seg_rows = fused_seg_rows(p.ny, cached_wps);
+      a.nsegs = (J_ - 3 + a.seg_rows - 1) / a.seg_rows;
+      a.seg_groups = (a.nsegs + kWavesPerBlock - 1) / kWavesPerBlock;
+      const dim3 mgrid(((S_ - 3 + kStripOut - 1) / kStripOut) * a.seg_groups);
+      if (hllc)
+        hipLaunchKernelGGL((k_flux_store<P, 2>), mgrid, dim3(64 * kWavesPerBlock), 0, st, a);
+      else if (minmod)
+        hipLaunchKernelGGL((k_flux_store<P, 0, true>), mgrid, dim3(64 * kWavesPerBlock), 0, st, a);
+      else
+        hipLaunchKernelGGL((k_flux_store<P, 0>), mgrid, dim3(64 * kWavesPerBlock), 0, st, a);
+    } else {
+      hipLaunchKernelGGL(k_slopes<P>, dim3((S_ - 2 + 63) / 64, (J_ - 2 + 3) / 4), dim3(256), 0, st, a);
+      hipLaunchKernelGGL(k_trace<P>, dim3((S_ - 2 + 63) / 64, (J_ - 2 + 3) / 4), dim3(256), 0, st, a);
+      if (p.riemann_solver == E2D_RIEMANN_HLLC)
+        hipLaunchKernelGGL((k_flux<P, 2>), fgrid, dim3(256), 0, st, a);
+      else
+        hipLaunchKernelGGL((k_flux<P, 0>), fgrid, dim3(256), 0, st, a);
+    }
+  }
+  {
+    const TraceRange range("updateHydro");
+    hipLaunchKernelGGL(k_update<P>, dim3((p.nx + 63) / 64, (p.ny + 3) / 4), dim3(256), 0, st, a);
+  }
   return hipGetLastError();
 }
 

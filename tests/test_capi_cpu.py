@@ -99,3 +99,17 @@ def test_namelist_reference_layout(tmp_path):
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(e2d.LibraryNotBuilt):
         e2d.load_library(tmp_path / "nope.so")
+
+
+def test_trace_ranges_linked():
+    """The library opens roctx ranges (recorded by rocprofv3 --marker-trace) named
+    after the reference's NVTX ranges (serial_ref/m_run.f90:195,264,336,387,502,571)."""
+    import subprocess
+    so = e2d.LIB_PATH
+    undef = subprocess.run(["nm", "-D", "--undefined-only", str(so)], capture_output=True, text=True,
+                           check=True).stdout
+    assert "roctxRangePushA" in undef and "roctxRangePop" in undef
+    data = so.read_bytes()
+    for name in (b"godunov_compute", b"compute_dt", b"convertToPrimitives", b"computeAndStoreFluxes",
+                 b"updateHydro", b"make_boundaries"):
+        assert name + b"\0" in data, name

@@ -49,6 +49,34 @@ def make_run(e2d, nx, impl, prec, problem="diag"):
     return run
 
 
+def settle_clock(e2d, nx, impl, prec, ms: float) -> int:
+    """Run steps of the same workload on a run of its own for `ms` ms of wall
+    time, untimed, and free it; returns the step count.  An idle MI355X starts
+    a launch series at a lower engine clock and raises it over the first tens
+    of milliseconds.  Measured on the fused FP64 step at 2048^2: its
+    GRBM_GUI_ACTIVE count stays at 3.8e6 cycles per launch while the launch
+    time falls from 219 to 200 us over the first 55 launches (2.23 to 2.39
+    GHz); back to back, launches take 214-231 us over the first 20 and
+    192 us from the 130th on.  The work per step is flat over those steps:
+    the mean Newton count per face of the diagonal problem is 1.004-1.011
+    from step 5 to step 200.  Settling first times the kernel at the clock a
+    long run holds; the timed run itself still starts from the initial
+    state."""
+    if ms <= 0:
+        return 0
+    run = make_run(e2d, nx, impl, prec)
+    n = 0
+    try:
+        t0 = time.perf_counter()
+        while (time.perf_counter() - t0) * 1e3 < ms:
+            run.run_steps(10)
+            run.synchronize()
+            n += 10
+    finally:
+        run.close()
+    return n
+
+
 def time_steps(run, steps: int, warmup: int) -> tuple[float, float]:
     """(seconds for `steps` timed steps, kernel-average ms from hipEvents)."""
     run.run_steps(warmup)
@@ -340,6 +368,9 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--settle-ms", type=float, default=250.0,
+                    help="before the warmup, run the workload untimed on a separate run for this many ms "
+                         "so the engine clock has settled (0: off)")
     ap.add_argument("--adjoint-steps", type=int, default=0,
                     help="only time this many adjoint steps (dense seeded lam, state after 100 steps) "
                          "against as many forward steps at the workload size and exit")
@@ -374,6 +405,7 @@ def main(argv=None):
     nx = args.nx
 
     if args.adjoint_steps:
+        settle_clock(e2d, nx, 0, prec, args.settle_ms)
         run = make_run(e2d, nx, 0, prec)
         run.run_steps(100)
         run.synchronize()
@@ -399,6 +431,7 @@ def main(argv=None):
 
     _log(f"[bench] {nx}x{nx} {args.precision} impl {args.impl}: {args.warmup} warmup + {args.steps} steps")
     run = make_run(e2d, nx, args.impl, prec)
+    settle_steps = settle_clock(e2d, nx, args.impl, prec, args.settle_ms)
     wall, kernel_ms = time_steps(run, args.steps, args.warmup)
     dt_final = run.get_dt()
     # copy ceiling: device-to-device copies of one state buffer through the
@@ -473,6 +506,8 @@ def main(argv=None):
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        # untimed steps of a separate run before the warmup (settle_clock)
+        "clock_settle": {"ms": args.settle_ms, "steps": settle_steps},
         "ms_per_step": round(ms_per_step, 5),
         "higher_is_better": True,
         "scaling": "weak",

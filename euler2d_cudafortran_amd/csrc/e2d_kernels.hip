@@ -45,6 +45,10 @@ constexpr int kStripOut = 60;  // output columns per wavefront
 #endif
 constexpr int kWavesPerBlock = E2D_WAVES_PER_BLOCK;
 
+#ifndef E2D_FUSED_UOLD_LDS
+#define E2D_FUSED_UOLD_LDS 1
+#endif
+
 // optional register cap for the fused kernel (minimum resident waves per SIMD)
 #if defined(E2D_FUSED_MIN_WAVES) && E2D_FUSED_MIN_WAVES > 0
 #define E2D_FUSED_ATTR __attribute__((amdgpu_waves_per_eu(E2D_FUSED_MIN_WAVES)))
@@ -69,7 +73,21 @@ struct StepArgs {
   int nsegs, seg_rows, nlong;
   int nshort, short_rows, short_base, nitems;
   int chunk;  // items per chunk dealt to one XCD (see fused_segment)
+  int nstrips, tile;  // items are numbered over tiles of `tile` adjacent strips (see tile_item)
 };
+
+// Item k of a tiled numbering of nstrips strips x nper segments each: tiles of
+// `tile` adjacent strips (the last one narrower), numbered tile by tile and
+// within a tile segment by segment, so that consecutive items are horizontal
+// neighbours first.  tile == 1 is the strip-major order.
+__device__ __forceinline__ void tile_item(int k, int nper, int nstrips, int tile, int& strip, int& seg) {
+  const int span = tile * nper;
+  const int tc = k / span;
+  const int w = min(tile, nstrips - tc * tile);
+  const int in = k - tc * span;
+  seg = in / w;
+  strip = tc * tile + (in - seg * w);
+}
 
 // Segment (strip, rows [j0, j1)) of wavefront `wave` of block b; false for a
 // wavefront without one.  Blocks go to the 8 XCDs round-robin (b % 8) and
@@ -85,13 +103,14 @@ __device__ __forceinline__ bool fused_segment(const StepArgs<P>& a, int b, int w
   int item;
   bool is_long;
   if (!dealt_item(b, wave, kWavesPerBlock, a.nlong, a.nitems, a.chunk, item, is_long)) return false;
+  int seg;
   if (is_long) {
-    strip = item / a.nsegs;
-    j0 = 2 + (item - strip * a.nsegs) * a.seg_rows;
+    tile_item(item, a.nsegs, a.nstrips, a.tile, strip, seg);
+    j0 = 2 + seg * a.seg_rows;
     j1 = min(j0 + a.seg_rows, a.short_base);
   } else {
-    strip = item / a.nshort;
-    j0 = a.short_base + (item - strip * a.nshort) * a.short_rows;
+    tile_item(item, a.nshort, a.nstrips, a.tile, strip, seg);
+    j0 = a.short_base + seg * a.short_rows;
     j1 = min(j0 + a.short_rows, a.g.ny + 2);
   }
   return true;
@@ -186,20 +205,36 @@ __device__ __forceinline__ void step_fused_body(const StepArgs<P>& a) {
   U fy_prev[4] = {U(0), U(0), U(0), U(0)}, fx_prev[4] = {U(0), U(0), U(0), U(0)};
   U fy_cur[4] = {U(0), U(0), U(0), U(0)}, fx_cur[4] = {U(0), U(0), U(0), U(0)};
 
+#if E2D_FUSED_UOLD_LDS
+  // The update of row r-1 adds to that row's input, loaded two iterations
+  // earlier: each lane keeps its last three input rows in LDS (slot of row k:
+  // k mod 3) instead of loading them again from memory, where the line may
+  // have left L2 by then.  A lane reads only what it wrote itself.
+  // (raw storage: S may have a constructor, which __shared__ does not allow)
+  __shared__ __align__(16) unsigned char rows_in_mem[kWavesPerBlock * 3 * 4 * 64 * sizeof(S)];
+  S* const rows_in = reinterpret_cast<S*>(rows_in_mem) + wave * (3 * 4 * 64);  // [slot][v][lane]
+  int slot_w = j0 % 3;  // slot of row r+1
+#endif
   for (int r = j0 - 1; r <= j1; ++r) {
     // software pipeline: row r+1 was fetched one iteration ago; fetch row r+2
-    // now and the input of the cell updated this iteration, so both loads are
-    // in flight during the Riemann solves
+    // now, so its load is in flight during the Riemann solves
 #pragma unroll
     for (int v = 0; v < 4; ++v) raw[v] = raw_next[v];
     if (r + 2 <= j1 + 1) fetch_cell<P>(a, cl, r + 2, raw_next);
     S uold[4];
     const bool do_update = (r >= j0 + 1);
+#if !E2D_FUSED_UOLD_LDS
     if (do_update && out_lane) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) uold[v] = a.uin[at(g, ci, r - 1, v)];
     }
+#endif
     source_cell<P>(a, cl, col_ok, r + 1, raw);
+#if E2D_FUSED_UOLD_LDS
+    // row r+1 as stored (interior cells carry no reflection flip)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) rows_in[(slot_w * 4 + v) * 64 + lane] = raw[v];
+#endif
     prim_of<P>(h, raw, qc);
 
     // slopes and trace sources of row r
@@ -241,6 +276,11 @@ __device__ __forceinline__ void step_fused_body(const StepArgs<P>& a) {
       for (int v = 0; v < 4; ++v) fxr[v] = from_right(fx_prev[v]);
       if (out_lane) {
         const int jr = r - 1;
+#if E2D_FUSED_UOLD_LDS
+        const int slot_r = slot_w == 2 ? 0 : slot_w + 1;  // (r - 1) mod 3
+#pragma unroll
+        for (int v = 0; v < 4; ++v) uold[v] = rows_in[(slot_r * 4 + v) * 64 + lane];
+#endif
         U un[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -264,6 +304,9 @@ __device__ __forceinline__ void step_fused_body(const StepArgs<P>& a) {
       fy_prev[v] = fy_cur[v];
       fx_prev[v] = fx_cur[v];
     }
+#if E2D_FUSED_UOLD_LDS
+    slot_w = slot_w == 2 ? 0 : slot_w + 1;
+#endif
   }
 
   if (a.track_dt) {
@@ -383,6 +426,8 @@ static int plan_segments(StepArgs<P>& a, int nstrips, int ny, int wps, int resid
   a.nshort = covered < ny ? (ny - covered + r - 1) / r : 0;
   a.nitems = a.nlong + nstrips * a.nshort;
   a.chunk = std::max(1, env_int("E2D_FUSED_CHUNK", 16));
+  a.nstrips = nstrips;
+  a.tile = std::min(std::max(1, env_int("E2D_FUSED_TILE", 1)), std::max(1, nstrips));
   return dealt_blocks(a.nlong, a.nitems, a.chunk, kWavesPerBlock);
 }
 

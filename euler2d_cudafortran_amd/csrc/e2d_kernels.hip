@@ -48,6 +48,9 @@ constexpr int kWavesPerBlock = E2D_WAVES_PER_BLOCK;
 #ifndef E2D_FUSED_UOLD_LDS
 #define E2D_FUSED_UOLD_LDS 1
 #endif
+#ifndef E2D_FUSED_PLAIN
+#define E2D_FUSED_PLAIN 1
+#endif
 
 // optional register cap for the fused kernel (minimum resident waves per SIMD)
 #if defined(E2D_FUSED_MIN_WAVES) && E2D_FUSED_MIN_WAVES > 0
@@ -122,17 +125,25 @@ __device__ __forceinline__ bool fused_segment(const StepArgs<P>& a, int b, int w
 // once.  source_cell applies the reflection's sign flip when the values are
 // used and writes ghost values back, so that the input buffer ends up as
 // make_boundaries leaves it; doing that here, right after the loads, would
-// make every load wait for its data.
+// make every load wait for its data.  plain (uniform over the wavefront):
+// every cell the wavefront loads is interior, so the boundary map is skipped.
 template <class P>
-__device__ __forceinline__ void fetch_cell(const StepArgs<P>& a, int ci, int r, typename P::S raw[4]) {
+__device__ __forceinline__ void fetch_cell(const StepArgs<P>& a, int ci, int r, bool plain,
+                                           typename P::S raw[4]) {
+  if (plain) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) raw[v] = a.uin[at(a.g, ci, r, v)];
+    return;
+  }
   const Src s = ghost_source(a.g, ci, r);
 #pragma unroll
   for (int v = 0; v < 4; ++v) raw[v] = a.uin[at(a.g, s.i, s.j, v)];
 }
 
 template <class P>
-__device__ __forceinline__ void source_cell(const StepArgs<P>& a, int ci, bool col_ok, int r,
+__device__ __forceinline__ void source_cell(const StepArgs<P>& a, int ci, bool col_ok, int r, bool plain,
                                             typename P::S raw[4]) {
+  if (plain) return;
   const Src s = ghost_source(a.g, ci, r);
 #pragma unroll
   for (int v = 0; v < 4; ++v) raw[v] = (v == s.flip) ? -raw[v] : raw[v];
@@ -170,6 +181,13 @@ __device__ __forceinline__ void step_fused_body(const StepArgs<P>& a) {
   const bool col_ok = ci < g.isize;
   const int cl = col_ok ? ci : g.isize - 1;  // clamped column for loads
   const bool out_lane = (lane >= 2) && (lane < 62) && (ci <= g.nx + 1);
+  // columns strip*60 .. strip*60+63 and rows j0-2 .. j1+1 all interior
+#if E2D_FUSED_PLAIN
+  const bool plain = __builtin_amdgcn_readfirstlane(strip * kStripOut >= 2 && strip * kStripOut + 63 <= g.nx + 1 &&
+                                                    j0 - 2 >= 2 && j1 + 1 <= g.ny + 1 ? 1 : 0) != 0;
+#else
+  const bool plain = false;
+#endif
 
   const Hydro<T>& h = a.hT;
   U rate_max = U(0);
@@ -180,12 +198,12 @@ __device__ __forceinline__ void step_fused_body(const StepArgs<P>& a) {
   if (a.dt_from_device) invdt_cur = invdt_part(a.scal, a.slot_cur);
   T qa[4], qb[4], qc[4];
   S raw[4], raw_next[4], raw_b[4];
-  fetch_cell<P>(a, cl, j0 - 2, raw);
-  fetch_cell<P>(a, cl, j0 - 1, raw_b);
-  fetch_cell<P>(a, cl, j0, raw_next);
-  source_cell<P>(a, cl, col_ok, j0 - 2, raw);
+  fetch_cell<P>(a, cl, j0 - 2, plain, raw);
+  fetch_cell<P>(a, cl, j0 - 1, plain, raw_b);
+  fetch_cell<P>(a, cl, j0, plain, raw_next);
+  source_cell<P>(a, cl, col_ok, j0 - 2, plain, raw);
   prim_of<P>(h, raw, qa);
-  source_cell<P>(a, cl, col_ok, j0 - 1, raw_b);
+  source_cell<P>(a, cl, col_ok, j0 - 1, plain, raw_b);
   prim_of<P>(h, raw_b, qb);
 
   // dt of this step (serial_ref/m_run.f90:308, 384-385)
@@ -220,7 +238,7 @@ __device__ __forceinline__ void step_fused_body(const StepArgs<P>& a) {
     // now, so its load is in flight during the Riemann solves
 #pragma unroll
     for (int v = 0; v < 4; ++v) raw[v] = raw_next[v];
-    if (r + 2 <= j1 + 1) fetch_cell<P>(a, cl, r + 2, raw_next);
+    if (r + 2 <= j1 + 1) fetch_cell<P>(a, cl, r + 2, plain, raw_next);
     S uold[4];
     const bool do_update = (r >= j0 + 1);
 #if !E2D_FUSED_UOLD_LDS
@@ -229,7 +247,7 @@ __device__ __forceinline__ void step_fused_body(const StepArgs<P>& a) {
       for (int v = 0; v < 4; ++v) uold[v] = a.uin[at(g, ci, r - 1, v)];
     }
 #endif
-    source_cell<P>(a, cl, col_ok, r + 1, raw);
+    source_cell<P>(a, cl, col_ok, r + 1, plain, raw);
 #if E2D_FUSED_UOLD_LDS
     // row r+1 as stored (interior cells carry no reflection flip)
 #pragma unroll

@@ -58,6 +58,10 @@ struct DualMath {
     const double s = Base::sqrt(a.v);
     return Dual(s, s == 0.0 ? 0.0 : a.d / (2.0 * s));
   }
+  static E2D_HDI Dual sqrt_pos(Dual a) {
+    const double s = Base::sqrt_pos(a.v);
+    return Dual(s, a.d / (2.0 * s));
+  }
 };
 
 template <class T>

@@ -62,6 +62,19 @@ int validate(const e2d_params* p) {
   if (!(p->gamma0 > 1.0)) return fail(E2D_ERR_ARG, "gamma0 must be > 1");
   if (!(p->cfl > 0.0)) return fail(E2D_ERR_ARG, "cfl must be > 0");
   if (!(p->smallr > 0.0) || !(p->smallc > 0.0)) return fail(E2D_ERR_ARG, "smallr and smallc must be > 0");
+  // The Riemann solver's radicands are bounded below by gamma0 smallr^2 smallp
+  // (1 - gamma6) (densities and pressures are floored at smallr and
+  // rho smallp, and p* >= 0); the step takes their square roots without a
+  // zero case, so that bound must be a positive normal double.
+  const double smallp = p->smallc * p->smallc / p->gamma0;
+  const double gamma6 = (p->gamma0 + 1.0) / (2.0 * p->gamma0);
+  const double radicand_floor = p->gamma0 * p->smallr * (p->smallr * smallp) * (1.0 - gamma6);
+  if (!(gamma6 < 1.0) || !(radicand_floor >= 1e-290))
+    return fail(E2D_ERR_ARG,
+                "gamma0 %g, smallr %g and smallc %g put the pressure and sound-speed floors below the "
+                "double range (gamma0 * smallr^2 * smallc^2 / gamma0 * (1 - (gamma0 + 1) / (2 gamma0)) = %g, "
+                "must be >= 1e-290)",
+                p->gamma0, p->smallr, p->smallc, radicand_floor);
   return E2D_OK;
 }
 

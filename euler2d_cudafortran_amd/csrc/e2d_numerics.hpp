@@ -45,16 +45,21 @@ struct Hydro {
 // Division / square root policy.  ExactMath keeps IEEE correctly rounded
 // operations (bitwise parity).  FastMath (float only) uses the native
 // reciprocal and square root (about 1 ulp), for the mixed-precision variant.
+// sqrt_pos(x) is sqrt(x) for a radicand that is never zero (positive or NaN):
+// a policy may skip its zero case there.
 struct ExactMath {
   static E2D_HDI double div(double a, double b) { return a / b; }
   static E2D_HDI float div(float a, float b) { return a / b; }
   static E2D_HDI double sqrt(double a) { return __builtin_sqrt(a); }
   static E2D_HDI float sqrt(float a) { return __builtin_sqrtf(a); }
+  static E2D_HDI double sqrt_pos(double a) { return __builtin_sqrt(a); }
+  static E2D_HDI float sqrt_pos(float a) { return __builtin_sqrtf(a); }
 };
 #if defined(__HIPCC__)
 struct FastMath {
   static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
   static __device__ __forceinline__ float sqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
+  static __device__ __forceinline__ float sqrt_pos(float a) { return __builtin_amdgcn_sqrtf(a); }
 };
 
 // RangedMath (FP64): correctly rounded division and square root for finite,
@@ -65,7 +70,9 @@ struct FastMath {
 // the range scaling (v_div_scale, ldexp by 2^256) and special-value fixup
 // (v_div_fixup, class tests) that act only outside that range.  Inside it the
 // results are the IEEE results bit for bit, except that a zero quotient is +0
-// whatever the sign of the numerator; a zero radicand returns zero.
+// whatever the sign of the numerator; a zero radicand returns zero (sqrt; the
+// core alone, sqrt_pos, returns NaN for it and is the IEEE result for every
+// positive radicand in the range).
 //
 // The FP32 overloads do the same for binary32 (exponents within about +-90,
 // the solver's 1e-22..1e9 again): division is the reciprocal seed, one Newton
@@ -91,6 +98,7 @@ struct RangedMath {
     t = __builtin_fmaf(-up, s, x) > 0.0f ? up : t;
     return t;
   }
+  static __device__ __forceinline__ float sqrt_pos(float x) { return sqrt(x); }
   static __device__ __forceinline__ double div(double a, double b) {
     const double nb = -b;
     double r = __builtin_amdgcn_rcp(b);
@@ -102,7 +110,8 @@ struct RangedMath {
     const double rem = __builtin_fma(nb, q, a);
     return __builtin_fma(rem, r, q);
   }
-  static __device__ __forceinline__ double sqrt(double x) {
+  static __device__ __forceinline__ double sqrt(double x) { return x == 0.0 ? x : sqrt_pos(x); }
+  static __device__ __forceinline__ double sqrt_pos(double x) {
     const double y = __builtin_amdgcn_rsq(x);
     double g = x * y;
     double hh = y * 0.5;
@@ -112,8 +121,7 @@ struct RangedMath {
     double d = __builtin_fma(-g, g, x);
     g = __builtin_fma(d, hh, g);
     d = __builtin_fma(-g, g, x);
-    const double s = __builtin_fma(d, hh, g);
-    return x == 0.0 ? x : s;
+    return __builtin_fma(d, hh, g);
   }
 };
 #endif
@@ -166,10 +174,12 @@ E2D_HDI void cons_to_prim(const Hydro<T>& h, const T u[4], T q[4]) {
   q[3] = vy;
 }
 
-// Sound speed of a primitive state (second half of eos).
+// Sound speed of a primitive state (second half of eos) as cons_to_prim
+// leaves it: q[0] >= smallr and q[1] >= q[0] * smallp, so the radicand is at
+// least gamma0 * smallp > 0.
 template <class T, class M>
 E2D_HDI T sound_speed(const Hydro<T>& h, const T q[4]) {
-  return M::sqrt(M::div(h.gamma0 * q[1], q[0]));
+  return M::sqrt_pos(M::div(h.gamma0 * q[1], q[0]));
 }
 
 // Local CFL rate (c+|u|)/dx + (c+|v|)/dy of a conservative state
@@ -290,17 +300,20 @@ E2D_HDI void two_shock_start(const Hydro<T>& h, const T ql[4], const T qr[4], Tw
   // Lagrangian sound speeds squared and first guess
   s.cl = h.gamma0 * s.pl * s.rl;
   s.cr = h.gamma0 * s.pr * s.rr;
-  const T wl = M::sqrt(s.cl);
-  const T wr = M::sqrt(s.cr);
+  // cl >= gamma0 * smallr^2 * smallp > 0: rl and pl are floored above
+  const T wl = M::sqrt_pos(s.cl);
+  const T wr = M::sqrt_pos(s.cr);
   s.p_star = fmax2(M::div((wr * s.pl + wl * s.pr) + wl * wr * (s.ul - s.ur), wl + wr), T(0));
   s.conv = T(1);
 }
 
 // one Newton iteration: the next p* and its convergence measure
+// (p* >= 0, so 1 + gamma6 (p* - p) / p >= 1 - gamma6 > 0 and the radicands
+// are at least cl (1 - gamma6) > 0)
 template <class T, class M>
 E2D_HDI void two_shock_newton(const Hydro<T>& h, const TwoShock<T>& s, T& p_next, T& conv_next) {
-  const T wwl = M::sqrt(s.cl * (T(1) + M::div(h.gamma6 * (s.p_star - s.pl), s.pl)));
-  const T wwr = M::sqrt(s.cr * (T(1) + M::div(h.gamma6 * (s.p_star - s.pr), s.pr)));
+  const T wwl = M::sqrt_pos(s.cl * (T(1) + M::div(h.gamma6 * (s.p_star - s.pl), s.pl)));
+  const T wwr = M::sqrt_pos(s.cr * (T(1) + M::div(h.gamma6 * (s.p_star - s.pr), s.pr)));
   const T zl = M::div(T(2) * wwl * wwl * wwl, wwl * wwl + s.cl);
   const T zr = M::div(T(2) * wwr * wwr * wwr, wwr * wwr + s.cr);
   const T usl = s.ul - M::div(s.p_star - s.pl, wwl);
@@ -314,8 +327,8 @@ template <class T, class M>
 E2D_HDI void two_shock_finish(const Hydro<T>& h, const T ql[4], const T qr[4], const TwoShock<T>& s, T g[4],
                               T f[4]) {
   const T rl = s.rl, ul = s.ul, pl = s.pl, rr = s.rr, ur = s.ur, pr = s.pr, p_star = s.p_star;
-  const T wl = M::sqrt(s.cl * (T(1) + M::div(h.gamma6 * (p_star - pl), pl)));
-  const T wr = M::sqrt(s.cr * (T(1) + M::div(h.gamma6 * (p_star - pr), pr)));
+  const T wl = M::sqrt_pos(s.cl * (T(1) + M::div(h.gamma6 * (p_star - pl), pl)));
+  const T wr = M::sqrt_pos(s.cr * (T(1) + M::div(h.gamma6 * (p_star - pr), pr)));
   const T u_star = T(0.5) * (ul + M::div(pl - p_star, wl) + ur - M::div(pr - p_star, wr));
 
   // upwind side of the contact (sgnm = +-1)
@@ -324,7 +337,8 @@ E2D_HDI void two_shock_finish(const Hydro<T>& h, const T ql[4], const T qr[4], c
   const T uo = right_going ? ul : ur;
   const T po = right_going ? pl : pr;
   const T wo = right_going ? wl : wr;
-  const T co = fmax2(h.smallc, M::sqrt(fabs_(M::div(h.gamma0 * po, ro))));
+  // po / ro >= smallp (the same side's floors); p* / r* may be zero
+  const T co = fmax2(h.smallc, M::sqrt_pos(fabs_(M::div(h.gamma0 * po, ro))));
   const T r_star = fmax2(M::div(ro, T(1) + M::div(ro * (po - p_star), wo * wo)), h.smallr);
   const T c_star = fmax2(h.smallc, M::sqrt(fabs_(M::div(h.gamma0 * p_star, r_star))));
 

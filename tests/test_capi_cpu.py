@@ -50,6 +50,9 @@ def _params(**kw):
     (dict(gamma0=1.0), "gamma0"),
     (dict(cfl=0.0), "cfl"),
     (dict(smallr=0.0), "smallr"),
+    (dict(smallc=1e-150), "floors below the double range"),
+    (dict(smallr=1e-100, smallc=1e-80), "floors below the double range"),
+    (dict(gamma0=1.0 + 1e-300), "gamma0"),
     (dict(niter_riemann=-1), "niter_riemann"),
 ])
 def test_create_rejects_bad_params(bad, msg):

@@ -219,7 +219,9 @@ E2D_HDI void godunov_flux_adj(const Hydro<T>& h, const T g[4], const T fb[4], T 
   gb[2] += mass_b * r;
 }
 
-// One Newton iteration of riemann_two_shock from p_k; returns p_{k+1}.
+// One Newton iteration of riemann_two_shock from p_k; returns p_{k+1}.  (The
+// square roots keep their zero case here: without it, the FP64 adjoint
+// kernel is scheduled 1.5% slower.)
 template <class M, class T>
 E2D_HDI T newton_step(const Hydro<T>& h, T pk, T cl, T cr, T pl, T pr,
                            T ul, T ur, T& conv, bool& take_t) {

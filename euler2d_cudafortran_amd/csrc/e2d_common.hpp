@@ -29,8 +29,10 @@ using P32 = Prec<float, float, float, RangedMath, RangedMath>;
 // FP32 state; FP32 flux arithmetic with the native reciprocal and square
 // root; the update added in FP64 and rounded to the FP32 state
 using PNative = Prec<float, float, double, FastMath, RangedMath>;
-// FP64 state and update, FP32 slope / trace / Riemann arithmetic
-using PMix = Prec<double, float, double, RangedMath, RangedMath>;
+// FP64 state and update, FP32 slope / trace / Riemann arithmetic with the
+// native reciprocal and square root (as fp32_native; the correctly rounded
+// binary32 cores cost 33% more time for 5-7% less error at 4096^2)
+using PMix = Prec<double, float, double, FastMath, RangedMath>;
 
 // f(P{}) for the policy P of an E2D_PREC_* code (validated by the caller)
 template <class F>

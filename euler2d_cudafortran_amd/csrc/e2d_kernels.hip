@@ -403,7 +403,7 @@ static int waves_per_simd(const void* kernel) {
 // launch.  Measured on MI355X (us per step, dt on the device): 2048^2 FP64
 // 19-row long segments (one round) 213.5 against 223 with 16 rows over 75%
 // (which leaves 9% of the long segments for a second round); FP32 12 rows
-// 134-136 against 139-140; mixed 106.3 against 107.1; 4096^2 FP64 37 rows
+// 134-136 against 139-140; fp32_native 106.3 against 107.1; 4096^2 FP64 37 rows
 // (two rounds) 711 against 716, 74 rows (one round) 719.  Where the rounds
 // would give segments of fewer than 8 rows (grids up to about 1024^2) the
 // plan is long segments of fused_seg_rows rows over 75% of the rows.
@@ -417,9 +417,9 @@ static int plan_segments(StepArgs<P>& a, int nstrips, int ny, int wps, int resid
   const int round = resident / std::max(1, nstrips);  // long segments per strip in one round
   if (!forced && round >= 1) {
     // the long segments' share of the rows: 80% for the FP64 kernels (3
-    // resident wavefronts per SIMD), 88% for the float ones (4-5): measured
-    // at 2048^2, FP64 210-213 us at 80% against 214-218 at 76, 84 and 88%;
-    // FP32 132.5 at 88% against 137 at 80%, mixed 104.5 against 107
+    // resident wavefronts per SIMD), 88% for the float-arithmetic ones (4-5):
+    // measured at 2048^2, FP64 210-213 us at 80% against 214-218 at 76, 84 and
+    // 88%; FP32 132.5 at 88% against 137 at 80%, fp32_native 104.5 against 107
     const int cover = env_int("E2D_FUSED_COVER_PCT", wps >= 4 ? 88 : 80);
     const double long_rows = std::min(100, std::max(10, cover)) / 100.0 * ny;
     int k = 1;

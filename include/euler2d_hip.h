@@ -51,7 +51,8 @@ extern "C" {
                                   root (about 1 ulp); update added in FP64 and rounded to the FP32 state;
                                   FP64 dt and sums */
 #define E2D_PREC_MIXED 3       /* FP64 state and FP64 update, FP32 slope/trace/Riemann arithmetic
-                                  (correctly rounded division and square root), FP64 dt and sums */
+                                  with the native reciprocal and square root (as fp32_native), FP64
+                                  dt and sums */
 
 /* riemann_solver: serial_ref/m_constants.f90:21-23 */
 #define E2D_RIEMANN_APPROX 0

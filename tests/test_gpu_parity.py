@@ -292,9 +292,9 @@ def test_sums_match_host(gpu_lib):
 
 # relative L1 after 200 steps of the 256^2 diagonal problem, per variable;
 # measured on MI355X: fp32_native 1.07e-7 6.88e-8 3.02e-7 3.02e-7, mixed
-# 3.18e-8 3.35e-8 1.33e-7 1.33e-7; bounds three times those
+# 4.15e-8 4.47e-8 1.71e-7 1.71e-7; bounds three times those
 FLOAT_VARIANT_BOUNDS = {e2d.PREC_FP32_NATIVE: [3.2e-7, 2.1e-7, 9.1e-7, 9.1e-7],
-                        e2d.PREC_MIXED: [9.6e-8, 1e-7, 4e-7, 4e-7]}
+                        e2d.PREC_MIXED: [1.25e-7, 1.35e-7, 5.2e-7, 5.2e-7]}
 
 
 @pytest.mark.parametrize("prec", [e2d.PREC_FP32_NATIVE, e2d.PREC_MIXED])

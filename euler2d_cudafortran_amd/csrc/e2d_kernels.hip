@@ -45,6 +45,9 @@ constexpr int kStripOut = 60;  // output columns per wavefront
 #endif
 constexpr int kWavesPerBlock = E2D_WAVES_PER_BLOCK;
 
+// 1: the fused step takes the update's input rows from an LDS ring (0: loads
+// them again from memory); 1: wavefronts that load only interior cells skip
+// the boundary map (0: every load goes through it).  Both give the same bits.
 #ifndef E2D_FUSED_UOLD_LDS
 #define E2D_FUSED_UOLD_LDS 1
 #endif
@@ -70,9 +73,9 @@ struct StepArgs {
   int dt_from_device, slot_cur, track_dt, slot_next;
   double dt_host;
   // Row segments, one per wavefront: items [0, nlong) are nsegs segments of
-  // seg_rows rows per strip (strip-major) over rows [2, short_base); items
-  // [nlong, nitems) are nshort segments of short_rows rows per strip over
-  // [short_base, ny + 2).
+  // seg_rows rows per strip over rows [2, short_base); items [nlong, nitems)
+  // are nshort segments of short_rows rows per strip over [short_base, ny + 2);
+  // each range is numbered by tile_item.
   int nsegs, seg_rows, nlong;
   int nshort, short_rows, short_base, nitems;
   int chunk;  // items per chunk dealt to one XCD (see fused_segment)

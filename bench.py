@@ -54,10 +54,10 @@ def settle_clock(e2d, nx, impl, prec, ms: float) -> int:
     time, untimed, and free it; returns the step count.  An idle MI355X starts
     a launch series at a lower engine clock and raises it over the first tens
     of milliseconds.  Measured on the fused FP64 step at 2048^2: its
-    GRBM_GUI_ACTIVE count stays at 3.7-3.9e6 cycles and its SQ_INSTS_VALU at
-    9.12-9.18e7 per launch while the launch time falls from 217 to 192 us
-    over the first 60 launches (2.22 to 2.44 GHz); back to back, launches
-    take 209-222 us over the first 25 and 188-189 us from the 130th on.  The
+    GRBM_GUI_ACTIVE count stays at 3.6-3.7e6 cycles and its SQ_INSTS_VALU at
+    8.76-8.82e7 per launch while the launch time falls from 208 to 185 us
+    over the first 50 launches (2.23 to 2.46 GHz); back to back, launches
+    take 202-214 us over the first 25 and 180-184 us from the 130th on.  The
     work per step is flat over those steps: the mean Newton count per face of
     the diagonal problem is 1.004-1.011 from step 5 to step 1000.  Settling
     first times the kernel at the clock a long run holds; the timed run

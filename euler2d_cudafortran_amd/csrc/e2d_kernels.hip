@@ -236,12 +236,24 @@ __device__ __forceinline__ void step_fused_body(const StepArgs<P>& a) {
   S* const rows_in = reinterpret_cast<S*>(rows_in_mem) + wave * (3 * 4 * 64);  // [slot][v][lane]
   int slot_w = j0 % 3;  // slot of row r+1
 #endif
-  for (int r = j0 - 1; r <= j1; ++r) {
+  // Variable 0 of row r+2 of the input and of row r-1 of the output in the
+  // lane's clamped column cl (ci on every lane that stores), moved one row on
+  // per iteration: no row-index product per access.
+  const S* in_row = a.uin + at(g, cl, j0 + 1, 0);
+  S* out_row = a.uout + at(g, cl, j0 - 2, 0);
+  for (int r = j0 - 1; r <= j1; ++r, in_row += g.pitch, out_row += g.pitch) {
     // software pipeline: row r+1 was fetched one iteration ago; fetch row r+2
     // now, so its load is in flight during the Riemann solves
 #pragma unroll
     for (int v = 0; v < 4; ++v) raw[v] = raw_next[v];
-    if (r + 2 <= j1 + 1) fetch_cell<P>(a, cl, r + 2, plain, raw_next);
+    if (r + 2 <= j1 + 1) {
+      if (plain) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) raw_next[v] = in_row[v * g.plane];
+      } else {
+        fetch_cell<P>(a, cl, r + 2, plain, raw_next);
+      }
+    }
     S uold[4];
     const bool do_update = (r >= j0 + 1);
 #if !E2D_FUSED_UOLD_LDS
@@ -307,7 +319,7 @@ __device__ __forceinline__ void step_fused_body(const StepArgs<P>& a) {
         for (int v = 0; v < 4; ++v) {
           const U flux_tot = (fy_prev[v] - fy_cur[v]) + (fx_prev[v] - fxr[v]);
           const S sn = static_cast<S>(static_cast<U>(uold[v]) + flux_tot);
-          a.uout[at(g, ci, jr, v)] = sn;
+          out_row[v * g.plane] = sn;
           un[v] = static_cast<U>(sn);
         }
         // CFL rate of the new state over the compute_dt window [2, n] (0-based)

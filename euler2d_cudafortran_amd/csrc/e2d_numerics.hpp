@@ -132,9 +132,10 @@ struct RangedMath {
 #endif
 
 // Fortran max/min as the reference compiler lowers them: a > b ? a : b and
-// a < b ? a : b.  In double device code they are the hardware v_max_f64 /
-// v_min_f64 (IEEE maxNum/minNum), which return the same bits except when b is
-// NaN or {a, b} = {+0, -0}.  Neither case changes a result on the FP64 path:
+// a < b ? a : b.  In device code they are the hardware v_max_f64 / v_min_f64
+// and v_max_f32 / v_min_f32 (IEEE maxNum/minNum), which return the same bits
+// except when b is NaN or {a, b} = {+0, -0}.  Neither case changes a result of
+// the step, in either precision:
 // every floor compares against a positive constant or rho*smallp > 0, the
 // limiter compares absolute values, max(x, 0) for the first pressure guess
 // never sees x = -0 (a sum with a positive term), and the Newton correction
@@ -147,6 +148,7 @@ template <class T>
 E2D_HDI T fmax2(T a, T b) {
 #if defined(__HIP_DEVICE_COMPILE__) && !E2D_SELECT_MINMAX
   if constexpr (std::is_same<T, double>::value) return __builtin_fmax(a, b);
+  if constexpr (std::is_same<T, float>::value) return __builtin_fmaxf(a, b);
 #endif
   return a > b ? a : b;
 }
@@ -154,6 +156,7 @@ template <class T>
 E2D_HDI T fmin2(T a, T b) {
 #if defined(__HIP_DEVICE_COMPILE__) && !E2D_SELECT_MINMAX
   if constexpr (std::is_same<T, double>::value) return __builtin_fmin(a, b);
+  if constexpr (std::is_same<T, float>::value) return __builtin_fminf(a, b);
 #endif
   return a < b ? a : b;
 }

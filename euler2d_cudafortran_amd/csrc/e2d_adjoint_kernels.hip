@@ -597,7 +597,7 @@ __device__ __forceinline__ void finish_cell_lam(const AdjArgs<P>& a, int ci, int
 }
 
 template <class P>
-__global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march_gather(const AdjArgs<P> a) {
+__device__ __forceinline__ void adj_march_gather_body(const AdjArgs<P>& a) {
   using T = typename P::T;
   using U = typename P::U;
   using M = typename P::M;
@@ -838,6 +838,22 @@ __global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march
       lam_p[v] = lam_r[v];
     }
   }
+}
+
+// The FP64 instantiation is held at 2 resident wavefronts per SIMD (the
+// E2D_ADJ_MARCH_MIN_WAVES cap).  The float-arithmetic ones are held at 3 (168
+// VGPRs): left to itself the compiler gives fp32_native and mixed 173-177
+// VGPRs, 2 wavefronts per SIMD, and their adjoint steps take 312 and 323 us
+// at 2048^2 against 284 and 303 us at 3 (with a 28-36 B spill); FP32 fits
+// in 163 either way.
+template <class P>
+__global__ __launch_bounds__(64 * kAdjWaves) E2D_ADJ_MARCH_ATTR void k_adj_march_gather(const AdjArgs<P> a) {
+  adj_march_gather_body<P>(a);
+}
+template <class P>
+__global__ __launch_bounds__(64 * kAdjWaves) __attribute__((amdgpu_waves_per_eu(3))) void k_adj_march_gather_w3(
+    const AdjArgs<P> a) {
+  adj_march_gather_body<P>(a);
 }
 
 // The rows k_adj_march_gather leaves unfinished: row 0, row jsize-1, and the
@@ -1113,7 +1129,10 @@ hipError_t adjoint_step(const e2d_params& p, const DevBuffers& b, int buf, doubl
   } else if (passes == Passes::kFused) {
     const int nstrips = (g.isize + kFusedStripOut - 1) / kFusedStripOut;
     const int nblocks = plan_adjoint_segments(a, nstrips, g.jsize - 2);
-    hipLaunchKernelGGL(k_adj_march_gather<P>, dim3(nblocks), dim3(64 * kAdjWaves), 0, st, a);
+    if constexpr (sizeof(T) == 4)
+      hipLaunchKernelGGL(k_adj_march_gather_w3<P>, dim3(nblocks), dim3(64 * kAdjWaves), 0, st, a);
+    else
+      hipLaunchKernelGGL(k_adj_march_gather<P>, dim3(nblocks), dim3(64 * kAdjWaves), 0, st, a);
     const int per_strip = a.nsegs + a.nshort;
     hipLaunchKernelGGL(k_adj_seam<P>, dim3((g.isize + 63) / 64, (2 + 2 * per_strip + 3) / 4), dim3(256), 0, st, a);
   } else {

@@ -62,6 +62,11 @@ struct DualMath {
     const double s = Base::sqrt_pos(a.v);
     return Dual(s, a.d / (2.0 * s));
   }
+  static E2D_HDI Dual sqrt_pos_inv(Dual a, Dual& inv) {
+    inv = Dual(0.0, 0.0);
+    return sqrt_pos(a);
+  }
+  static E2D_HDI Dual div_inv(Dual a, Dual b, Dual) { return div(a, b); }
 };
 
 template <class T>

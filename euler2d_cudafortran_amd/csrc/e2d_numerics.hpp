@@ -46,7 +46,10 @@ struct Hydro {
 // operations (bitwise parity).  FastMath (float only) uses the native
 // reciprocal and square root (about 1 ulp), for the mixed-precision variant.
 // sqrt_pos(x) is sqrt(x) for a radicand that is never zero (positive or NaN):
-// a policy may skip its zero case there.
+// a policy may skip its zero case there.  sqrt_pos_inv(x, inv) is
+// sqrt_pos(x) that may also leave in inv a reciprocal of the root which
+// div_inv(a, s, inv) = div(a, s) may use in place of its own (the same
+// quotient); policies without one leave inv unused.
 struct ExactMath {
   static E2D_HDI double div(double a, double b) { return a / b; }
   static E2D_HDI float div(float a, float b) { return a / b; }
@@ -54,12 +57,24 @@ struct ExactMath {
   static E2D_HDI float sqrt(float a) { return __builtin_sqrtf(a); }
   static E2D_HDI double sqrt_pos(double a) { return __builtin_sqrt(a); }
   static E2D_HDI float sqrt_pos(float a) { return __builtin_sqrtf(a); }
+  template <class T>
+  static E2D_HDI T sqrt_pos_inv(T a, T& inv) {
+    inv = T(0);
+    return sqrt_pos(a);
+  }
+  template <class T>
+  static E2D_HDI T div_inv(T a, T b, T) { return div(a, b); }
 };
 #if defined(__HIPCC__)
 struct FastMath {
   static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
   static __device__ __forceinline__ float sqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
   static __device__ __forceinline__ float sqrt_pos(float a) { return __builtin_amdgcn_sqrtf(a); }
+  static __device__ __forceinline__ float sqrt_pos_inv(float a, float& inv) {
+    inv = 0.0f;
+    return sqrt_pos(a);
+  }
+  static __device__ __forceinline__ float div_inv(float a, float b, float) { return div(a, b); }
 };
 
 // RangedMath (FP64): correctly rounded division and square root for finite,
@@ -99,6 +114,11 @@ struct RangedMath {
     return t;
   }
   static __device__ __forceinline__ float sqrt_pos(float x) { return sqrt(x); }
+  static __device__ __forceinline__ float sqrt_pos_inv(float x, float& inv) {
+    inv = 0.0f;
+    return sqrt(x);
+  }
+  static __device__ __forceinline__ float div_inv(float a, float b, float) { return div(a, b); }
   static __device__ __forceinline__ double div(double a, double b) {
     const double nb = -b;
     double r = __builtin_amdgcn_rcp(b);
@@ -117,9 +137,32 @@ struct RangedMath {
     return x == 0.0 ? x : s;
   }
   static __device__ __forceinline__ double sqrt_pos(double x) {
+    double hh;
+    return sqrt_core(x, hh);
+  }
+  // The Goldschmidt step leaves hh = 1/(2 sqrt(x)) to about 2^-45; 2 hh, one
+  // Newton step against the rounded root s (error squared to about 2^-90, then
+  // the step's rounding), is a reciprocal of s as good as div's two Newton
+  // steps from v_rcp_f64, without the quarter-rate v_rcp_f64.
+  static __device__ __forceinline__ double sqrt_pos_inv(double x, double& inv) {
+    double hh;
+    const double s = sqrt_core(x, hh);
+    const double r0 = hh + hh;
+    inv = __builtin_fma(r0, __builtin_fma(-s, r0, 1.0), r0);
+    return s;
+  }
+  // div's remainder correction with the reciprocal r of b given
+  static __device__ __forceinline__ double div_inv(double a, double b, double r) {
+    const double q = a * r;
+    const double rem = __builtin_fma(-b, q, a);
+    return __builtin_fma(rem, r, q);
+  }
+
+ private:
+  static __device__ __forceinline__ double sqrt_core(double x, double& hh) {
     const double y = __builtin_amdgcn_rsq(x);
     double g = x * y;
-    double hh = y * 0.5;
+    hh = y * 0.5;
     const double r = __builtin_fma(-hh, g, 0.5);
     g = __builtin_fma(g, r, g);
     hh = __builtin_fma(hh, r, hh);
@@ -320,12 +363,13 @@ E2D_HDI void two_shock_start(const Hydro<T>& h, const T ql[4], const T qr[4], Tw
 // are at least cl (1 - gamma6) > 0)
 template <class T, class M>
 E2D_HDI void two_shock_newton(const Hydro<T>& h, const TwoShock<T>& s, T& p_next, T& conv_next) {
-  const T wwl = M::sqrt_pos(s.cl * (T(1) + M::div(h.gamma6 * (s.p_star - s.pl), s.pl)));
-  const T wwr = M::sqrt_pos(s.cr * (T(1) + M::div(h.gamma6 * (s.p_star - s.pr), s.pr)));
+  T inv_l, inv_r;
+  const T wwl = M::sqrt_pos_inv(s.cl * (T(1) + M::div(h.gamma6 * (s.p_star - s.pl), s.pl)), inv_l);
+  const T wwr = M::sqrt_pos_inv(s.cr * (T(1) + M::div(h.gamma6 * (s.p_star - s.pr), s.pr)), inv_r);
   const T zl = M::div(T(2) * wwl * wwl * wwl, wwl * wwl + s.cl);
   const T zr = M::div(T(2) * wwr * wwr * wwr, wwr * wwr + s.cr);
-  const T usl = s.ul - M::div(s.p_star - s.pl, wwl);
-  const T usr = s.ur + M::div(s.p_star - s.pr, wwr);
+  const T usl = s.ul - M::div_inv(s.p_star - s.pl, wwl, inv_l);
+  const T usr = s.ur + M::div_inv(s.p_star - s.pr, wwr, inv_r);
   const T delp = fmax2(M::div(zr * zl, zr + zl) * (usl - usr), -s.p_star);
   p_next = s.p_star + delp;
   conv_next = fabs_(M::div(delp, p_next + h.smallpp));
@@ -335,9 +379,10 @@ template <class T, class M>
 E2D_HDI void two_shock_finish(const Hydro<T>& h, const T ql[4], const T qr[4], const TwoShock<T>& s, T g[4],
                               T f[4]) {
   const T rl = s.rl, ul = s.ul, pl = s.pl, rr = s.rr, ur = s.ur, pr = s.pr, p_star = s.p_star;
-  const T wl = M::sqrt_pos(s.cl * (T(1) + M::div(h.gamma6 * (p_star - pl), pl)));
-  const T wr = M::sqrt_pos(s.cr * (T(1) + M::div(h.gamma6 * (p_star - pr), pr)));
-  const T u_star = T(0.5) * (ul + M::div(pl - p_star, wl) + ur - M::div(pr - p_star, wr));
+  T inv_l, inv_r;
+  const T wl = M::sqrt_pos_inv(s.cl * (T(1) + M::div(h.gamma6 * (p_star - pl), pl)), inv_l);
+  const T wr = M::sqrt_pos_inv(s.cr * (T(1) + M::div(h.gamma6 * (p_star - pr), pr)), inv_r);
+  const T u_star = T(0.5) * (ul + M::div_inv(pl - p_star, wl, inv_l) + ur - M::div_inv(pr - p_star, wr, inv_r));
 
   // upwind side of the contact (sgnm = +-1)
   const bool right_going = u_star >= T(0);

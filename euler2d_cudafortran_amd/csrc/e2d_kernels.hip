@@ -1302,6 +1302,12 @@ __global__ __launch_bounds__(256) void k_pointwise(const Hydro<typename P::T> h,
     case E2D_FN_SQRT:
       y[k] = M::sqrt(static_cast<T>(x[k]));
       break;
+    case E2D_FN_DIV_ROOT: {
+      T inv;
+      const T s = M::sqrt_pos_inv(static_cast<T>(x[2 * k + 1]), inv);
+      y[2 * k] = M::div_inv(static_cast<T>(x[2 * k]), s, inv);
+      y[2 * k + 1] = s;
+    } break;
     default: break;
   }
 }

@@ -44,7 +44,8 @@ struct Hydro {
 
 // Division / square root policy.  ExactMath keeps IEEE correctly rounded
 // operations (bitwise parity).  FastMath (float only) uses the native
-// reciprocal and square root (about 1 ulp), for the mixed-precision variant.
+// reciprocal and square root (quotients within 2 ulp, roots within 1 ulp), for
+// the fp32_native and mixed variants.
 // sqrt_pos(x) is sqrt(x) for a radicand that is never zero (positive or NaN):
 // a policy may skip its zero case there.  sqrt_pos_inv(x, inv) is
 // sqrt_pos(x) that may also leave in inv a reciprocal of the root which

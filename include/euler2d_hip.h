@@ -48,8 +48,8 @@ extern "C" {
 #define E2D_PREC_FP64 0        /* FP64 state and arithmetic (serial_ref built with fp_kind = dp_kind) */
 #define E2D_PREC_FP32 1        /* FP32 state and arithmetic (serial_ref default kind) */
 #define E2D_PREC_FP32_NATIVE 2 /* FP32 state; FP32 flux arithmetic with the native reciprocal and square
-                                  root (about 1 ulp); update added in FP64 and rounded to the FP32 state;
-                                  FP64 dt and sums */
+                                  root (quotients within 2 ulp, roots within 1 ulp); update added in
+                                  FP64 and rounded to the FP32 state; FP64 dt and sums */
 #define E2D_PREC_MIXED 3       /* FP64 state and FP64 update, FP32 slope/trace/Riemann arithmetic
                                   with the native reciprocal and square root (as fp32_native), FP64
                                   dt and sums */
@@ -85,6 +85,9 @@ extern "C" {
 #define E2D_FN_EOS 7     /* rho, eint                 -> p, c               */
 #define E2D_FN_DIV 8     /* a, b                      -> a / b (the precision's flux arithmetic) */
 #define E2D_FN_SQRT 9    /* x                         -> sqrt(x)            */
+#define E2D_FN_DIV_ROOT 10 /* a, x                    -> a / sqrt(x), sqrt(x) for x > 0, formed as the
+                              two-shock solver forms (p* - p) / w: the quotient may take its
+                              reciprocal from the square root's own iteration */
 
 /*
  * Run parameters: the namelist groups &RUN/&MESH/&HYDRO/&OTHER of the

@@ -116,3 +116,24 @@ def test_trace_ranges_linked():
     for name in (b"godunov_compute", b"compute_dt", b"convertToPrimitives", b"computeAndStoreFluxes",
                  b"updateHydro", b"make_boundaries"):
         assert name + b"\0" in data, name
+
+
+def test_binding_constants_match_the_header():
+    """Every E2D_IMPL_/PREC_/RIEMANN_/PROBLEM_/STAGE_/FN_ value of
+    include/euler2d_hip.h has the same value in the Python binding, and
+    each pointwise function has its widths there."""
+    import euler2d_cudafortran_amd.hydro as hy
+
+    text = (ROOT / "include" / "euler2d_hip.h").read_text()
+    defs = {m.group(1): int(m.group(2)) for m in re.finditer(r"^#define E2D_(\w+) \(?(-?\d+)\)?", text, re.M)}
+    checked = 0
+    for name, value in defs.items():
+        group = name.split("_")[0]
+        if group in ("IMPL", "PREC", "RIEMANN", "PROBLEM", "FN"):
+            assert getattr(hy, name) == value, name
+            checked += 1
+        elif group == "STAGE":
+            assert hy.STAGES[name[len("STAGE_"):].lower()] == value, name
+            checked += 1
+    fns = {v for k, v in defs.items() if k.startswith("FN_")}
+    assert fns == set(hy.FN_WIDTH) and checked >= 30

@@ -7,12 +7,14 @@ IEEE sequences).  Inside the solver's range (magnitudes 1e-22..1e9) they must
 return the correctly rounded result: numpy's float64 / float32 division and
 sqrt, bit for bit (a zero may differ in sign).  Operands: log-uniform over the
 range with both signs for numerators, plus mantissa edge cases (powers of two,
-all-ones mantissas, quotients next to 1, perfect squares and their neighbours).
+all-ones mantissas, quotients next to 1, perfect squares and their neighbours),
+and the hard rounding cases of tests/hard_cases.py.
 """
 import numpy as np
 import pytest
 
 import euler2d_cudafortran_amd as e2d
+import hard_cases as hc
 
 pytestmark = pytest.mark.gpu
 
@@ -64,3 +66,77 @@ def test_sqrt_core_is_correctly_rounded(gpu_lib, prec, dt):
     want = np.sqrt(x)
     bad = np.nonzero(got != want)[0]
     assert bad.size == 0, f"{bad.size} roots differ, first: {x[bad[:3]]} -> {got[bad[:3]]} vs {want[bad[:3]]}"
+
+
+# Hard rounding cases (tests/hard_cases.py): exact quotients and roots within a
+# tiny fraction of an ulp of a midpoint, spread over the solver's magnitudes.
+HARD_DIV = 1 << 17
+HARD_SQRT = 1 << 15
+
+
+@pytest.mark.parametrize("prec, dt, p", [(e2d.PREC_FP64, np.float64, 53), (e2d.PREC_FP32, np.float32, 24)])
+def test_division_core_hard_cases(gpu_lib, prec, dt, p):
+    a, b = hc.scaled_div_operands(hc.div_near_midpoint(HARD_DIV, p, seed=21), p, seed=22)
+    p_ = e2d.HydroParams(nx=8, ny=8, precision=prec)
+    got = e2d.pointwise(p_, e2d.FN_DIV, np.stack([a, b], axis=1))[:, 0].astype(dt)
+    want = a.astype(dt) / b.astype(dt)
+    bad = np.nonzero(got != want)[0]
+    assert bad.size == 0, f"{bad.size} of {a.size} hard quotients differ, first: {a[bad[:3]]} / {b[bad[:3]]}"
+
+
+@pytest.mark.parametrize("prec, dt, p", [(e2d.PREC_FP64, np.float64, 53), (e2d.PREC_FP32, np.float32, 24)])
+def test_sqrt_core_hard_cases(gpu_lib, prec, dt, p):
+    x = hc.scaled_radicands(hc.sqrt_near_midpoint(HARD_SQRT, p, seed=23), p, seed=24)
+    p_ = e2d.HydroParams(nx=8, ny=8, precision=prec)
+    got = e2d.pointwise(p_, e2d.FN_SQRT, x)[:, 0].astype(dt)
+    want = np.sqrt(x.astype(dt))
+    bad = np.nonzero(got != want)[0]
+    assert bad.size == 0, f"{bad.size} of {x.size} hard roots differ, first: {x[bad[:3]]}"
+
+
+@pytest.mark.parametrize("prec, dt, p", [(e2d.PREC_FP64, np.float64, 53), (e2d.PREC_FP32, np.float32, 24)])
+def test_division_by_root_is_correctly_rounded(gpu_lib, prec, dt, p):
+    """a / sqrt(x) as the two-shock solver forms (p* - p) / w (E2D_FN_DIV_ROOT):
+    in FP64 the quotient takes its reciprocal from the square root's own
+    Goldschmidt iteration instead of a fresh one; root and quotient must still be
+    the IEEE results, a / RN(sqrt(x)) rounded once."""
+    rng = np.random.default_rng(25)
+    a = _log_uniform(rng, N, -22, 9, np.float64, signed=True)
+    x = _log_uniform(rng, N, -22, 12, np.float64)
+    hx = np.concatenate([_log_uniform(rng, HARD_DIV, -22, 12, np.float64),
+                         hc.scaled_radicands(hc.sqrt_near_midpoint(HARD_SQRT, p, seed=26), p, seed=27)])
+    ha, hx = hc.div_by_root_operands(hx, p, seed=28)
+    a = np.concatenate([a, ha]).astype(dt)
+    x = np.concatenate([x, hx]).astype(dt)
+    p_ = e2d.HydroParams(nx=8, ny=8, precision=prec)
+    y = e2d.pointwise(p_, e2d.FN_DIV_ROOT, np.stack([a, x], axis=1).astype(np.float64)).astype(dt)
+    s = np.sqrt(x)
+    bad_s = np.nonzero(y[:, 1] != s)[0]
+    assert bad_s.size == 0, f"{bad_s.size} roots differ, first: {x[bad_s[:3]]}"
+    want = a / s
+    bad = np.nonzero(y[:, 0] != want)[0]
+    assert bad.size == 0, (f"{bad.size} of {a.size} quotients differ ({ha.size} hard), first: "
+                           f"{a[bad[:3]]} / sqrt({x[bad[:3]]})")
+
+
+def _ulps(got, want):
+    """|got - want| in units of the last place of want (float32)."""
+    spacing = np.abs(np.spacing(want.astype(np.float32))).astype(np.float64)
+    return np.abs(got.astype(np.float64) - want.astype(np.float64)) / spacing
+
+
+def test_native_float_arithmetic_error(gpu_lib):
+    """fp32_native's native reciprocal and square root: v_rcp_f32 then one
+    multiplication for a / b, v_sqrt_f32 for sqrt, each within about an ulp."""
+    rng = np.random.default_rng(29)
+    a = _log_uniform(rng, N, -22, 9, np.float32, signed=True)
+    b = _log_uniform(rng, N, -22, 9, np.float32, signed=True)
+    x = _log_uniform(rng, N, -22, 12, np.float32)
+    p_ = e2d.HydroParams(nx=8, ny=8, precision=e2d.PREC_FP32_NATIVE)
+    q = e2d.pointwise(p_, e2d.FN_DIV, np.stack([a, b], axis=1).astype(np.float64))[:, 0]
+    r = e2d.pointwise(p_, e2d.FN_SQRT, x.astype(np.float64))[:, 0]
+    dq = _ulps(q, a / b)
+    dr = _ulps(r, np.sqrt(x))
+    print(f"fp32_native: division max {dq.max():.3f} ulp ({np.mean(dq > 0):.3f} inexact), "
+          f"sqrt max {dr.max():.3f} ulp ({np.mean(dr > 0):.3f} inexact)")
+    assert dq.max() <= 2.0 and dr.max() <= 1.0

@@ -67,6 +67,9 @@ struct DualMath {
     return sqrt_pos(a);
   }
   static E2D_HDI Dual div_inv(Dual a, Dual b, Dual) { return div(a, b); }
+  static E2D_HDI Dual recip(Dual) { return Dual(0.0, 0.0); }
+  static E2D_HDI Dual recip_near(Dual, Dual) { return Dual(0.0, 0.0); }
+  static E2D_HDI bool quot_gt(Dual a, Dual d, Dual tol, Dual h) { return Base::quot_gt(a.v, d.v, tol.v, h.v); }
 };
 
 template <class T>
@@ -86,6 +89,8 @@ E2D_HDI Hydro<Dual> dual_hydro(const Hydro<T>& h) {
   r.inv_dy = h.inv_dy;
   r.dxy_pow2 = h.dxy_pow2;
   r.conv_tol = h.conv_tol;
+  r.inv_smallr = h.inv_smallr;
+  r.conv_half_ulp = h.conv_half_ulp;
   r.slope_type = h.slope_type;
   r.niter = h.niter;
   r.riemann_solver = h.riemann_solver;

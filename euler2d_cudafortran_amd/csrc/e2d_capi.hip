@@ -75,6 +75,12 @@ int validate(const e2d_params* p) {
                 "double range (gamma0 * smallr^2 * smallc^2 / gamma0 * (1 - (gamma0 + 1) / (2 gamma0)) = %g, "
                 "must be >= 1e-290)",
                 p->gamma0, p->smallr, p->smallc, radicand_floor);
+  // The Newton stop test compares |delp| - conv_tol d with half an ulp of
+  // conv_tol times d, d >= smallr smallp; that product must stay a normal
+  // double (half an ulp of the FP32 literal 1e-6 is 2^-73).
+  if (!(p->smallr * smallp >= 0x1p-949))
+    return fail(E2D_ERR_ARG, "smallr %g and smallc %g put the Newton stop test's pressure floor "
+                "smallr * smallc^2 / gamma0 = %g below 2^-949", p->smallr, p->smallc, p->smallr * smallp);
   return E2D_OK;
 }
 
@@ -973,13 +979,13 @@ int e2d_run_steps_noalloc(const e2d_params* p, void* d_u, void* d_u2, int64_t pi
 int e2d_pointwise(const e2d_params* p, int func, int n, const double* x, double* y) {
   g_err[0] = '\0';
   if (!p) return fail(E2D_ERR_ARG, "params is NULL");
-  if (func < E2D_FN_PRIM || func > E2D_FN_DIV_ROOT) return fail(E2D_ERR_ARG, "unknown function %d", func);
+  if (func < E2D_FN_PRIM || func > E2D_FN_CONV_GT) return fail(E2D_ERR_ARG, "unknown function %d", func);
   if (n < 0 || (n > 0 && (!x || !y))) return fail(E2D_ERR_ARG, "bad batch");
   if (p->precision < E2D_PREC_FP64 || p->precision > E2D_PREC_MIXED) return fail(E2D_ERR_ARG, "bad precision");
   if (n == 0) return E2D_OK;
   if (!e2d_device_available()) return fail(E2D_ERR_HIP, "no HIP device available");
-  static const int nin[E2D_FN_DIV_ROOT + 1] = {0, 4, 20, 15, 8, 8, 4, 2, 2, 1, 2};
-  static const int nout[E2D_FN_DIV_ROOT + 1] = {0, 5, 8, 4, 8, 8, 4, 2, 1, 1, 2};
+  static const int nin[E2D_FN_CONV_GT + 1] = {0, 4, 20, 15, 8, 8, 4, 2, 2, 1, 2, 3, 2};
+  static const int nout[E2D_FN_CONV_GT + 1] = {0, 5, 8, 4, 8, 8, 4, 2, 1, 1, 2, 1, 1};
   double *dx = nullptr, *dy = nullptr;
   HIP_TRY(hipMalloc(reinterpret_cast<void**>(&dx), sizeof(double) * n * nin[func]));
   if (hipMalloc(reinterpret_cast<void**>(&dy), sizeof(double) * n * nout[func]) != hipSuccess) {

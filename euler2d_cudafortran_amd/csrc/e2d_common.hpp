@@ -5,7 +5,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
+#include <limits>
+#include <type_traits>
 
 #include "e2d_adjoint.hpp"
 #include "e2d_internal.hpp"
@@ -64,6 +67,12 @@ inline Hydro<T> make_hydro(const e2d_params& p) {
   h.inv_dx = static_cast<T>(p.nx);
   h.inv_dy = static_cast<T>(p.ny);
   h.conv_tol = static_cast<T>(1e-6f);
+  h.inv_smallr = T(1) / h.smallr;
+  {  // in the scalar type of T (double for the tangent's dual numbers)
+    using R = std::conditional_t<std::is_same_v<T, float>, float, double>;
+    const R tol = static_cast<R>(1e-6f);
+    h.conv_half_ulp = static_cast<T>(std::ldexp(R(1), std::ilogb(tol) - std::numeric_limits<R>::digits));
+  }
   h.slope_type = p.slope_type;
   h.niter = p.niter_riemann;
   h.riemann_solver = p.riemann_solver;

@@ -1308,6 +1308,13 @@ __global__ __launch_bounds__(256) void k_pointwise(const Hydro<typename P::T> h,
       y[2 * k] = M::div_inv(static_cast<T>(x[2 * k]), s, inv);
       y[2 * k + 1] = s;
     } break;
+    case E2D_FN_DIV_NEAR: {
+      const T b = static_cast<T>(x[3 * k + 1]);
+      y[k] = M::div_inv(static_cast<T>(x[3 * k]), b, M::recip_near(b, static_cast<T>(x[3 * k + 2])));
+    } break;
+    case E2D_FN_CONV_GT:
+      y[k] = M::quot_gt(static_cast<T>(x[2 * k]), static_cast<T>(x[2 * k + 1]), h.conv_tol, h.conv_half_ulp) ? 1.0 : 0.0;
+      break;
     default: break;
   }
 }

@@ -39,6 +39,8 @@ struct Hydro {
   T inv_dx, inv_dy;
   int dxy_pow2;
   T conv_tol;  // Newton stop threshold: the FP32 literal 1e-6 widened to T (m_utils.f90:396)
+  T inv_smallr;  // 1 / smallr rounded: a starting reciprocal for quotients by a floored density
+  T conv_half_ulp;  // half an ulp of conv_tol: conv_tol + conv_half_ulp is the rounding boundary above it
   int slope_type, niter, riemann_solver;
 };
 
@@ -50,7 +52,11 @@ struct Hydro {
 // a policy may skip its zero case there.  sqrt_pos_inv(x, inv) is
 // sqrt_pos(x) that may also leave in inv a reciprocal of the root which
 // div_inv(a, s, inv) = div(a, s) may use in place of its own (the same
-// quotient); policies without one leave inv unused.
+// quotient); policies without one leave inv unused.  recip(b) is such a
+// reciprocal of b, and recip_near(b, r0) one from a start r0 within about
+// 2^-50 of 1/b (one Newton step); both return 0 where div_inv ignores them.
+// quot_gt(a, d, tol, h) is div(a, d) > tol for a >= 0 (or NaN) and d > 0,
+// h being half an ulp of tol; a policy may decide it without the quotient.
 struct ExactMath {
   static E2D_HDI double div(double a, double b) { return a / b; }
   static E2D_HDI float div(float a, float b) { return a / b; }
@@ -65,6 +71,12 @@ struct ExactMath {
   }
   template <class T>
   static E2D_HDI T div_inv(T a, T b, T) { return div(a, b); }
+  template <class T>
+  static E2D_HDI T recip(T) { return T(0); }
+  template <class T>
+  static E2D_HDI T recip_near(T, T) { return T(0); }
+  template <class T>
+  static E2D_HDI bool quot_gt(T a, T d, T tol, T) { return div(a, d) > tol; }
 };
 #if defined(__HIPCC__)
 struct FastMath {
@@ -76,6 +88,9 @@ struct FastMath {
     return sqrt_pos(a);
   }
   static __device__ __forceinline__ float div_inv(float a, float b, float) { return div(a, b); }
+  static __device__ __forceinline__ float recip(float) { return 0.0f; }
+  static __device__ __forceinline__ float recip_near(float, float) { return 0.0f; }
+  static __device__ __forceinline__ bool quot_gt(float a, float d, float tol, float) { return div(a, d) > tol; }
 };
 
 // RangedMath (FP64): correctly rounded division and square root for finite,
@@ -120,16 +135,36 @@ struct RangedMath {
     return sqrt(x);
   }
   static __device__ __forceinline__ float div_inv(float a, float b, float) { return div(a, b); }
-  static __device__ __forceinline__ double div(double a, double b) {
+  static __device__ __forceinline__ float recip(float) { return 0.0f; }
+  static __device__ __forceinline__ float recip_near(float, float) { return 0.0f; }
+  static __device__ __forceinline__ bool quot_gt(float a, float d, float tol, float) { return div(a, d) > tol; }
+  static __device__ __forceinline__ double div(double a, double b) { return div_inv(a, b, recip(b)); }
+  // the reciprocal of the division sequence: seed and two Newton steps
+  static __device__ __forceinline__ double recip(double b) {
     const double nb = -b;
     double r = __builtin_amdgcn_rcp(b);
     double e = __builtin_fma(nb, r, 1.0);
     r = __builtin_fma(r, e, r);
     e = __builtin_fma(nb, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    const double q = a * r;
-    const double rem = __builtin_fma(nb, q, a);
-    return __builtin_fma(rem, r, q);
+    return __builtin_fma(r, e, r);
+  }
+  // one Newton step from r0 = 1/b (1 + e0): error e0^2 and the step's
+  // rounding, as close as recip(b) once |e0| is below about 2^-50
+  static __device__ __forceinline__ double recip_near(double b, double r0) {
+    return __builtin_fma(r0, __builtin_fma(-b, r0, 1.0), r0);
+  }
+  // Rounded to nearest, a / d exceeds tol exactly when a / d > mu = tol + h,
+  // the midpoint between tol and the next double; with d > 0 that is
+  // x = a - tol d > h d.  q = fma(-tol, d, a) is x rounded once, and h d is
+  // exact (h is a power of two; the caller keeps h d a normal double), so
+  // q > h d implies x > h d and q < h d implies x < h d.  q == h d cannot
+  // happen: write d = B 2^b and mu = M 2^e (B < 2^53, M odd with 54 bits,
+  // h = 2^e, so h d has the ulp 2^(e+b)).  x - h d = a - mu d is not zero
+  // (a would need M B's 54+ bits) and, where a is near mu d, a multiple of
+  // 2^(e+b), so x lies at least one ulp of h d away from h d and does not
+  // round to it.  The order is decided without the quotient.
+  static __device__ __forceinline__ bool quot_gt(double a, double d, double tol, double h) {
+    return __builtin_fma(-tol, d, a) > h * d;
   }
   // (the core first, then the select: written as x == 0 ? x : sqrt_pos(x) the
   // compiler branches around the core, which costs the adjoint kernel 14%)
@@ -338,7 +373,8 @@ E2D_HDI void godunov_flux(const Hydro<T>& h, const T g[4], T f[4]) {
 // first guess; one Newton iteration; star state, sampling and flux.
 template <class T>
 struct TwoShock {
-  T rl, ul, pl, rr, ur, pr, cl, cr, p_star, conv;
+  T rl, ul, pl, rr, ur, pr, cl, cr, p_star;
+  bool more;  // the convergence measure |delp| / (p* + smallpp) still above conv_tol
 };
 
 template <class T, class M>
@@ -356,14 +392,15 @@ E2D_HDI void two_shock_start(const Hydro<T>& h, const T ql[4], const T qr[4], Tw
   const T wl = M::sqrt_pos(s.cl);
   const T wr = M::sqrt_pos(s.cr);
   s.p_star = fmax2(M::div((wr * s.pl + wl * s.pr) + wl * wr * (s.ul - s.ur), wl + wr), T(0));
-  s.conv = T(1);
+  s.more = true;  // the reference starts from conv = 1 > conv_tol
 }
 
-// one Newton iteration: the next p* and its convergence measure
-// (p* >= 0, so 1 + gamma6 (p* - p) / p >= 1 - gamma6 > 0 and the radicands
+// one Newton iteration: the next p* and whether its convergence measure
+// |delp / (p* + smallpp)| is still above conv_tol (p* >= 0, so the divisor
+// is positive; 1 + gamma6 (p* - p) / p >= 1 - gamma6 > 0 and the radicands
 // are at least cl (1 - gamma6) > 0)
 template <class T, class M>
-E2D_HDI void two_shock_newton(const Hydro<T>& h, const TwoShock<T>& s, T& p_next, T& conv_next) {
+E2D_HDI void two_shock_newton(const Hydro<T>& h, const TwoShock<T>& s, T& p_next, bool& more) {
   T inv_l, inv_r;
   const T wwl = M::sqrt_pos_inv(s.cl * (T(1) + M::div(h.gamma6 * (s.p_star - s.pl), s.pl)), inv_l);
   const T wwr = M::sqrt_pos_inv(s.cr * (T(1) + M::div(h.gamma6 * (s.p_star - s.pr), s.pr)), inv_r);
@@ -373,7 +410,7 @@ E2D_HDI void two_shock_newton(const Hydro<T>& h, const TwoShock<T>& s, T& p_next
   const T usr = s.ur + M::div_inv(s.p_star - s.pr, wwr, inv_r);
   const T delp = fmax2(M::div(zr * zl, zr + zl) * (usl - usr), -s.p_star);
   p_next = s.p_star + delp;
-  conv_next = fabs_(M::div(delp, p_next + h.smallpp));
+  more = M::quot_gt(fabs_(delp), p_next + h.smallpp, h.conv_tol, h.conv_half_ulp);
 }
 
 template <class T, class M>
@@ -391,15 +428,26 @@ E2D_HDI void two_shock_finish(const Hydro<T>& h, const T ql[4], const T qr[4], c
   const T uo = right_going ? ul : ur;
   const T po = right_going ? pl : pr;
   const T wo = right_going ? wl : wr;
+  // 1/ro serves three quotients.  1/wo^2 starts from (1/wo)^2 and 1/r* from
+  // (1 + E)/ro (r* = ro / (1 + E) unless floored; 1 + E lies between
+  // (gamma0 - 1)/(gamma0 + 1) and its inverse): each start is within a few
+  // ulp, and one Newton step makes it a reciprocal as close as recip's, so
+  // the quotients are the same bits.
+  const T rro = M::recip(ro);
   // po / ro >= smallp (the same side's floors); p* / r* may be zero
-  const T co = fmax2(h.smallc, M::sqrt_pos(fabs_(M::div(h.gamma0 * po, ro))));
-  const T r_star = fmax2(M::div(ro, T(1) + M::div(ro * (po - p_star), wo * wo)), h.smallr);
-  const T c_star = fmax2(h.smallc, M::sqrt(fabs_(M::div(h.gamma0 * p_star, r_star))));
+  const T co = fmax2(h.smallc, M::sqrt_pos(fabs_(M::div_inv(h.gamma0 * po, ro, rro))));
+  const T wo2 = wo * wo;
+  const T iwo = right_going ? inv_l : inv_r;
+  const T D = T(1) + M::div_inv(ro * (po - p_star), wo2, M::recip_near(wo2, iwo * iwo));
+  const T r_raw = M::div(ro, D);
+  const T r_star = fmax2(r_raw, h.smallr);
+  const T ir0 = r_raw > h.smallr ? D * rro : h.inv_smallr;
+  const T c_star = fmax2(h.smallc, M::sqrt(fabs_(M::div_inv(h.gamma0 * p_star, r_star, M::recip_near(r_star, ir0)))));
 
   const T s_uo = signed_by(right_going, uo);
   T spout = co - s_uo;
   T spin = c_star - signed_by(right_going, u_star);
-  const T ushock = M::div(wo, ro) - s_uo;
+  const T ushock = M::div_inv(wo, ro, rro) - s_uo;
   if (p_star >= po) {
     spin = ushock;
     spout = ushock;
@@ -428,7 +476,7 @@ E2D_HDI void riemann_two_shock(const Hydro<T>& h, const T ql[4], const T qr[4],
                                                   T g[4], T f[4]) {
   TwoShock<T> s;
   two_shock_start<T, M>(h, ql, qr, s);
-  for (int it = 0; it < h.niter && s.conv > h.conv_tol; ++it) two_shock_newton<T, M>(h, s, s.p_star, s.conv);
+  for (int it = 0; it < h.niter && s.more; ++it) two_shock_newton<T, M>(h, s, s.p_star, s.more);
   two_shock_finish<T, M>(h, ql, qr, s, g, f);
 }
 

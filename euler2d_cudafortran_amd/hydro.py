@@ -25,10 +25,12 @@ PREC_FP64, PREC_FP32, PREC_FP32_NATIVE, PREC_MIXED = 0, 1, 2, 3
 RIEMANN_APPROX, RIEMANN_HLLC = 0, 2
 PROBLEM_IMPLODE, PROBLEM_DIAG, PROBLEM_BUMPS = 0, 1, 2
 STAGES = {"q": 0, "fx": 1, "fy": 2, "dqx": 3, "dqy": 4, "qxm": 5, "qxp": 6, "qym": 7, "qyp": 8}
-FN_PRIM, FN_SLOPE, FN_TRACE, FN_RIEMANN, FN_HLLC, FN_CMPFLX, FN_EOS, FN_DIV, FN_SQRT, FN_DIV_ROOT = range(1, 11)
+(FN_PRIM, FN_SLOPE, FN_TRACE, FN_RIEMANN, FN_HLLC, FN_CMPFLX, FN_EOS, FN_DIV, FN_SQRT, FN_DIV_ROOT,
+ FN_DIV_NEAR, FN_CONV_GT) = range(1, 13)
 FN_WIDTH = {FN_PRIM: (4, 5), FN_SLOPE: (20, 8), FN_TRACE: (15, 4), FN_RIEMANN: (8, 8),
             FN_HLLC: (8, 8), FN_CMPFLX: (4, 4), FN_EOS: (2, 2), FN_DIV: (2, 1), FN_SQRT: (1, 1),
-            FN_DIV_ROOT: (2, 2)}
+            FN_DIV_ROOT: (2, 2), FN_DIV_NEAR: (3, 1),
+            FN_CONV_GT: (2, 1)}
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "fp32_native": PREC_FP32_NATIVE, "mixed": PREC_MIXED}
 PROBLEMS = {"implode": PROBLEM_IMPLODE, "diag": PROBLEM_DIAG, "bumps": PROBLEM_BUMPS}
 

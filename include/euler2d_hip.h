@@ -88,6 +88,12 @@ extern "C" {
 #define E2D_FN_DIV_ROOT 10 /* a, x                    -> a / sqrt(x), sqrt(x) for x > 0, formed as the
                               two-shock solver forms (p* - p) / w: the quotient may take its
                               reciprocal from the square root's own iteration */
+#define E2D_FN_DIV_NEAR 11 /* a, b, r0                -> a / b with the reciprocal of b refined from the
+                              start r0 (within about 2^-50 of 1/b), as the two-shock solver's
+                              finish forms its quotients by wo^2 and by r* */
+#define E2D_FN_CONV_GT 12 /* a, d                     -> 1 if a / d, rounded in the precision's arithmetic,
+                              exceeds the Newton tolerance conv_tol, else 0: the two-shock solver's
+                              stop test, a = |delp| >= 0, d = p* + smallpp > 0 */
 
 /*
  * Run parameters: the namelist groups &RUN/&MESH/&HYDRO/&OTHER of the

@@ -31,6 +31,8 @@ static Hydro<double> hydro(int slope_type) {
   h.inv_dx = h.inv_dy = 64.0;
   h.dxy_pow2 = 1;
   h.conv_tol = static_cast<double>(1e-6f);
+  h.inv_smallr = 1.0 / h.smallr;
+  h.conv_half_ulp = std::ldexp(1.0, std::ilogb(h.conv_tol) - 53);
   h.slope_type = slope_type;
   h.niter = 10;
   h.riemann_solver = 0;
@@ -223,6 +225,8 @@ int main() {
     hf.inv_dx = hf.inv_dy = static_cast<float>(h.inv_dx);
     hf.dxy_pow2 = h.dxy_pow2;
     hf.conv_tol = 1e-6f;
+    hf.inv_smallr = 1.0f / hf.smallr;
+    hf.conv_half_ulp = std::ldexp(1.0f, std::ilogb(hf.conv_tol) - 24);
     hf.slope_type = h.slope_type;
     hf.niter = h.niter;
     hf.riemann_solver = 0;

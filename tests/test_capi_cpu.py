@@ -53,6 +53,7 @@ def _params(**kw):
     (dict(smallc=1e-150), "floors below the double range"),
     (dict(smallr=1e-100, smallc=1e-80), "floors below the double range"),
     (dict(gamma0=1.0 + 1e-300), "gamma0"),
+    (dict(smallr=1e10, smallc=1e-150), "Newton stop test"),
     (dict(niter_riemann=-1), "niter_riemann"),
 ])
 def test_create_rejects_bad_params(bad, msg):

@@ -140,3 +140,48 @@ def test_native_float_arithmetic_error(gpu_lib):
     print(f"fp32_native: division max {dq.max():.3f} ulp ({np.mean(dq > 0):.3f} inexact), "
           f"sqrt max {dr.max():.3f} ulp ({np.mean(dr > 0):.3f} inexact)")
     assert dq.max() <= 2.0 and dr.max() <= 1.0
+
+
+def test_division_from_a_near_reciprocal_is_correctly_rounded(gpu_lib):
+    """a / b with the reciprocal of b refined by one Newton step from a start r0
+    within about 2^-50 of 1/b (E2D_FN_DIV_NEAR), as the FP64 two-shock finish
+    forms its quotients by wo^2 (start (1/wo)^2) and by r* (start (1 + E)/ro):
+    the IEEE quotient for every start offset up to 8 ulp, on random and hard
+    operands."""
+    rng = np.random.default_rng(31)
+    a = _log_uniform(rng, N, -22, 9, np.float64, signed=True)
+    b = _log_uniform(rng, N, -22, 9, np.float64, signed=True)
+    ha, hb = hc.scaled_div_operands(hc.div_near_midpoint(HARD_DIV, 53, seed=32), 53, seed=33)
+    a, b = np.concatenate([a, ha]), np.concatenate([b, hb])
+    k = rng.integers(-8, 9, a.size).astype(np.float64)
+    r0 = (1.0 / b) * (1.0 + k * 2.0 ** -53)
+    p_ = e2d.HydroParams(nx=8, ny=8, precision=e2d.PREC_FP64)
+    got = e2d.pointwise(p_, e2d.FN_DIV_NEAR, np.stack([a, b, r0], axis=1))[:, 0]
+    want = a / b
+    bad = np.nonzero(got != want)[0]
+    assert bad.size == 0, f"{bad.size} of {a.size} quotients differ, first: {a[bad[:3]]} / {b[bad[:3]]} from {r0[bad[:3]]}"
+
+
+@pytest.mark.parametrize("prec, dt", [(e2d.PREC_FP64, np.float64), (e2d.PREC_FP32, np.float32)])
+def test_newton_stop_test_is_exact(gpu_lib, prec, dt):
+    """The two-shock solver's stop test, |delp| / (p* + smallpp) > conv_tol
+    (E2D_FN_CONV_GT).  FP64 decides it with one fma and a comparison against
+    the rounding boundary above conv_tol instead of forming the quotient; the
+    decision must be that of the IEEE quotient, on quotients a tiny fraction
+    of an ulp either side of that boundary and on random ones around conv_tol."""
+    tol = float(np.float32(1e-6))
+    rng = np.random.default_rng(35)
+    d = _log_uniform(rng, N, -8, 6, np.float64)
+    a = d * tol * (1.0 + rng.uniform(-1e-6, 1e-6, N))
+    a[: N // 8] = 0.0
+    a, d = a.astype(dt).astype(np.float64), d.astype(dt).astype(np.float64)
+    want = (a.astype(dt) / d.astype(dt)) > dt(tol)
+    if dt is np.float64:
+        ha, hd, above = hc.quotients_next_to_midpoint(tol, HARD_DIV, seed=36)
+        assert np.array_equal((ha / hd) > tol, above)
+        a, d, want = np.concatenate([a, ha]), np.concatenate([d, hd]), np.concatenate([want, above])
+    p_ = e2d.HydroParams(nx=8, ny=8, precision=prec)
+    got = e2d.pointwise(p_, e2d.FN_CONV_GT, np.stack([a, d], axis=1))[:, 0] == 1.0
+    bad = np.nonzero(got != want)[0]
+    assert 0.2 < want.mean() < 0.6
+    assert bad.size == 0, f"{bad.size} of {a.size} decisions differ, first: {a[bad[:3]]} / {d[bad[:3]]}"

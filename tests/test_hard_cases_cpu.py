@@ -71,3 +71,17 @@ def test_division_by_root_cases_sit_next_to_midpoints(p, dt):
         mid = (int(k) | 1) if int(k) % 2 == 1 else int(k) + 1
         nearest_mid = min((mid - 2, mid, mid + 2), key=lambda j: abs(k - j))
         assert abs(k - nearest_mid) < Fraction(1, 1 << (p - 6))
+
+
+def test_stop_test_cases_sit_next_to_the_tolerance_midpoint():
+    tol = float(np.float32(1e-6))
+    a, d, above = hc.quotients_next_to_midpoint(tol, 2000, seed=7)
+    h = np.spacing(tol) / 2
+    mu = Fraction(tol) + Fraction(h)
+    assert Fraction(np.nextafter(tol, 1.0)) - Fraction(tol) == 2 * Fraction(h)
+    for av, dv, up in zip(a.tolist(), d.tolist(), above.tolist()):
+        q = Fraction(av) / Fraction(dv)
+        # within 2^-70 of mu relative: 2^-17 of an ulp from the rounding boundary
+        assert (q > mu) == up and abs(q - mu) < mu * Fraction(1, 1 << 70)
+        assert ((av / dv) > tol) == up
+    assert 0.3 < above.mean() < 0.7

@@ -234,7 +234,7 @@ E2D_HDI void godunov_flux_adj(const Hydro<T>& h, const T g[4], const T fb[4], T 
 // kernel is scheduled 1.5% slower.)
 template <class M, class T>
 E2D_HDI T newton_step(const Hydro<T>& h, T pk, T cl, T cr, T pl, T pr,
-                           T ul, T ur, T& conv, bool& take_t) {
+                           T ul, T ur, bool& more, bool& take_t) {
   const T wwl = M::sqrt(cl * (T(1.0) + M::div(h.gamma6 * (pk - pl), pl)));
   const T wwr = M::sqrt(cr * (T(1.0) + M::div(h.gamma6 * (pk - pr), pr)));
   const T zl = M::div(T(2.0) * wwl * wwl * wwl, wwl * wwl + cl);
@@ -245,7 +245,7 @@ E2D_HDI T newton_step(const Hydro<T>& h, T pk, T cl, T cr, T pl, T pr,
   take_t = t > -pk;
   const T delp = take_t ? t : -pk;
   const T pn = pk + delp;
-  conv = fabs_(M::div(delp, pn + h.smallpp));
+  more = M::quot_gt(fabs_(delp), pn + h.smallpp, h.conv_tol, h.conv_half_ulp);
   return pn;
 }
 
@@ -330,11 +330,12 @@ E2D_HDI void riemann_two_shock_adj(const Hydro<T>& h, const T ql[4], const T qr[
   const T den = wl0 + wr0;
   const T p0r = M::div(num, den);
   const T p0 = fmax2(p0r, T(0.0));
-  T ps = p0, conv = T(1.0);
+  T ps = p0;
+  bool more = true;
   int K = 0;
-  for (; K < h.niter && conv > h.conv_tol; ++K) {
+  for (; K < h.niter && more; ++K) {
     bool tt;
-    ps = newton_step<M>(h, ps, cl, cr, pl, pr, ul, ur, conv, tt);
+    ps = newton_step<M>(h, ps, cl, cr, pl, pr, ul, ur, more, tt);
   }
   const T xlf = M::div(g6 * (ps - pl), pl), xrf = M::div(g6 * (ps - pr), pr);
   const T wl = M::sqrt(cl * (T(1.0) + xlf));
@@ -503,8 +504,8 @@ E2D_HDI void riemann_two_shock_adj(const Hydro<T>& h, const T ql[4], const T qr[
   // Newton iterations, last to first (p_k recomputed from p_0)
   T p_b = ps_b;
   for (int k = K - 1; k >= 0; --k) {
-    T pk = p0, cv;
-    bool tt;
+    T pk = p0;
+    bool cv, tt;
     for (int j = 0; j < k; ++j) pk = newton_step<M>(h, pk, cl, cr, pl, pr, ul, ur, cv, tt);
     p_b = newton_step_adj<M>(h, pk, cl, cr, pl, pr, ul, ur, p_b, cl_b, cr_b, pl_b, pr_b, ul_b, ur_b);
   }

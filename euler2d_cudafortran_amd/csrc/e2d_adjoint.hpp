@@ -197,7 +197,7 @@ E2D_HDI void face_state_adj(const Hydro<T>& h, const T q[4], const T dq[4], cons
                             T sb[4]) {
   const T half_dq0 = T(0.5) * dq[0];
   const T edge0 = plus_side ? q[0] + half_dq0 : q[0] - half_dq0;
-  const T f0raw = edge0 + s[0] * dtd * T(0.5);
+  const T f0raw = edge0 + s[0] * (dtd * T(0.5));  // as face_state forms it
   for (int k = 0; k < 4; ++k) {
     const T b = (k == 0 && h.smallr > f0raw) ? T(0.0) : fb[k];
     qb[k] += b;

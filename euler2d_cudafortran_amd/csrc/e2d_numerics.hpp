@@ -350,7 +350,9 @@ E2D_HDI void face_state(const Hydro<T>& h, const T q[4], const T dq[4],
   for (int v = 0; v < 4; ++v) {
     const T half_dq = T(0.5) * dq[v];
     const T edge = plus_side ? q[v] + half_dq : q[v] - half_dq;
-    f[v] = edge + s[v] * dtd * T(0.5);
+    // s (dtd / 2) is (s dtd) / 2 rounded alike (halving is exact in the
+    // solver's range), with dtd / 2 formed once
+    f[v] = edge + s[v] * (dtd * T(0.5));
   }
   f[0] = fmax2(h.smallr, f[0]);
 }

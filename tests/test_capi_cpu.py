@@ -81,6 +81,12 @@ def test_null_arguments_rejected():
     cp = _params()
     assert lib.e2d_pointwise(ctypes.byref(cp), 99, 1, None, None) == -1
     assert lib.e2d_pointwise(ctypes.byref(cp), 1, 0, None, None) == 0
+    # the function codes end at the last one the header declares
+    last = max(e2d.hydro.FN_WIDTH)
+    assert lib.e2d_pointwise(ctypes.byref(cp), last + 1, 1, None, None) == -1
+    assert "unknown function" in lib.e2d_last_error().decode()
+    assert lib.e2d_pointwise(ctypes.byref(cp), 0, 1, None, None) == -1
+    assert lib.e2d_pointwise(ctypes.byref(cp), last, 0, None, None) == 0
 
 
 def test_namelist_reference_layout(tmp_path):

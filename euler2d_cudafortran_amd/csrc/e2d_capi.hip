@@ -981,7 +981,8 @@ int e2d_pointwise(const e2d_params* p, int func, int n, const double* x, double*
   if (!p) return fail(E2D_ERR_ARG, "params is NULL");
   if (func < E2D_FN_PRIM || func > E2D_FN_CONV_GT) return fail(E2D_ERR_ARG, "unknown function %d", func);
   if (n < 0 || (n > 0 && (!x || !y))) return fail(E2D_ERR_ARG, "bad batch");
-  if (p->precision < E2D_PREC_FP64 || p->precision > E2D_PREC_MIXED) return fail(E2D_ERR_ARG, "bad precision");
+  // the device routines assume the floors e2d_create checks
+  if (int rc = validate(p)) return rc;
   if (n == 0) return E2D_OK;
   if (!e2d_device_available()) return fail(E2D_ERR_HIP, "no HIP device available");
   static const int nin[E2D_FN_CONV_GT + 1] = {0, 4, 20, 15, 8, 8, 4, 2, 2, 1, 2, 3, 2};

@@ -131,7 +131,12 @@ const char *e2d_last_error(void);
  * (cuf/m_HydroParameters_gpu.cuf:67-103, cuf/m_HydroRun_gpu.cuf:83-120):
  * validates p, allocates the two state buffers (u = buffer 0, u2 = buffer 1)
  * and, for implementation versions 1 and 2, the stage arrays.  Buffer 1 starts at
- * zero, as the reference's never-initialised u2 (serial_ref/m_run.f90:155). */
+ * zero, as the reference's never-initialised u2 (serial_ref/m_run.f90:155).
+ * Besides the ranges of the enumerations, p needs gamma0 > 1, cfl > 0,
+ * smallr > 0, smallc > 0 and floors that stay normal doubles:
+ * gamma0 smallr^2 smallp (1 - gamma6) >= 1e-290 (the Riemann solver's
+ * radicands) and smallr smallp >= 2^-949 (its Newton stop test), with
+ * smallp = smallc^2 / gamma0 and gamma6 = (gamma0 + 1) / (2 gamma0). */
 int e2d_create(const e2d_params *p, e2d_ctx **out);
 
 /* Replaces cleanupHydroRun (cuf/m_HydroRun_gpu.cuf:123-140). */
@@ -291,7 +296,8 @@ int e2d_copy_from_buffer(e2d_ctx *ctx, int buf, void *d_dst);
 /* Pointwise device numerics over a batch, for known-answer tests of the
  * device routines against the reference's (serial_ref/m_utils.f90).
  * x holds n items of the input width of func, y receives n items of its
- * output width (see E2D_FN_*); float64 host arrays; synchronous. */
+ * output width (see E2D_FN_*); float64 host arrays; synchronous.  p is
+ * validated as by e2d_create. */
 int e2d_pointwise(const e2d_params *p, int func, int n, const double *x, double *y);
 
 #ifdef __cplusplus

@@ -137,7 +137,13 @@ struct RangedMath {
   static __device__ __forceinline__ float div_inv(float a, float b, float) { return div(a, b); }
   static __device__ __forceinline__ float recip(float) { return 0.0f; }
   static __device__ __forceinline__ float recip_near(float, float) { return 0.0f; }
-  static __device__ __forceinline__ bool quot_gt(float a, float d, float tol, float) { return div(a, d) > tol; }
+  // binary32: the same test as the double one below, evaluated in double,
+  // where tol d is exact (48 bits), a - tol d is exact wherever it is near
+  // h d (about 49 bits), and h d is exact and normal for every binary32 d
+  static __device__ __forceinline__ bool quot_gt(float a, float d, float tol, float h) {
+    const double dd = d;
+    return __builtin_fma(-static_cast<double>(tol), dd, static_cast<double>(a)) > static_cast<double>(h) * dd;
+  }
   static __device__ __forceinline__ double div(double a, double b) { return div_inv(a, b, recip(b)); }
   // the reciprocal of the division sequence: seed and two Newton steps
   static __device__ __forceinline__ double recip(double b) {

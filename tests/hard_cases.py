@@ -119,19 +119,19 @@ def div_by_root_operands(x, p, seed, max_r=15):
     return np.array(a_out, dtype=np.float64), np.array(x_out, dtype=np.float64)
 
 
-def quotients_next_to_midpoint(tol, n, seed, max_r=1 << 20):
-    """(a, d, above): float64 pairs whose exact quotient a / d lies next to the
-    midpoint mu = tol + h between the double tol and the next one up (h half an
-    ulp of tol), above it where `above` is true.  With mu = M / 2^53 * 2^E
-    (M odd, 54 bits, E the exponent of tol) and an odd r, B = -r / M (mod
-    2^53) and A = (M B + r) / 2^53 give A / B = M / 2^53 + r / (2^53 B); then
-    a = A 2^(E+s), d = B 2^s for a random scale s keep that distance."""
+def quotients_next_to_midpoint(tol, n, seed, max_r=1 << 20, p=53):
+    """(a, d, above): float64 pairs, p-bit numbers, whose exact quotient a / d
+    lies next to the midpoint mu = tol + h between the p-bit tol and the next
+    one up (h half an ulp of tol), above it where `above` is true.  With
+    mu = M / 2^p * 2^E (M odd, p + 1 bits, E the exponent of tol) and an odd r,
+    B = -r / M (mod 2^p) and A = (M B + r) / 2^p give A / B = M / 2^p + r / (2^p B);
+    then a = A 2^(E+s), d = B 2^s for a random scale s keep that distance."""
     rng = random.Random(seed)
     m_tol, e_tol = np.frexp(tol)
-    tol_int = int(np.ldexp(m_tol, 53))          # tol = tol_int * 2^(e_tol - 53)
-    big_m = 2 * tol_int + 1                      # mu = big_m * 2^(e_tol - 54)
-    exp = int(e_tol) - 1                         # mu = big_m / 2^53 * 2^exp
-    mod = 1 << 53
+    tol_int = int(np.ldexp(m_tol, p))           # tol = tol_int * 2^(e_tol - p)
+    big_m = 2 * tol_int + 1                      # mu = big_m * 2^(e_tol - p - 1)
+    exp = int(e_tol) - 1                         # mu = big_m / 2^p * 2^exp
+    mod = 1 << p
     inv_m = pow(big_m, -1, mod)
     a_out, d_out, above = [], [], []
     while len(a_out) < n:

@@ -165,10 +165,11 @@ def test_division_from_a_near_reciprocal_is_correctly_rounded(gpu_lib):
 @pytest.mark.parametrize("prec, dt", [(e2d.PREC_FP64, np.float64), (e2d.PREC_FP32, np.float32)])
 def test_newton_stop_test_is_exact(gpu_lib, prec, dt):
     """The two-shock solver's stop test, |delp| / (p* + smallpp) > conv_tol
-    (E2D_FN_CONV_GT).  FP64 decides it with one fma and a comparison against
-    the rounding boundary above conv_tol instead of forming the quotient; the
-    decision must be that of the IEEE quotient, on quotients a tiny fraction
-    of an ulp either side of that boundary and on random ones around conv_tol."""
+    (E2D_FN_CONV_GT).  FP64 and FP32 decide it with one fma (in double) and a
+    comparison against the rounding boundary above conv_tol instead of forming
+    the quotient; the decision must be that of the IEEE quotient in the
+    precision, on quotients a tiny fraction of an ulp either side of that
+    boundary and on random ones around conv_tol."""
     tol = float(np.float32(1e-6))
     rng = np.random.default_rng(35)
     d = _log_uniform(rng, N, -8, 6, np.float64)
@@ -176,10 +177,10 @@ def test_newton_stop_test_is_exact(gpu_lib, prec, dt):
     a[: N // 8] = 0.0
     a, d = a.astype(dt).astype(np.float64), d.astype(dt).astype(np.float64)
     want = (a.astype(dt) / d.astype(dt)) > dt(tol)
-    if dt is np.float64:
-        ha, hd, above = hc.quotients_next_to_midpoint(tol, HARD_DIV, seed=36)
-        assert np.array_equal((ha / hd) > tol, above)
-        a, d, want = np.concatenate([a, ha]), np.concatenate([d, hd]), np.concatenate([want, above])
+    p = 53 if dt is np.float64 else 24
+    ha, hd, above = hc.quotients_next_to_midpoint(tol, HARD_DIV, seed=36, max_r=1 << (20 if p == 53 else 6), p=p)
+    assert np.array_equal((ha.astype(dt) / hd.astype(dt)) > dt(tol), above)
+    a, d, want = np.concatenate([a, ha]), np.concatenate([d, hd]), np.concatenate([want, above])
     p_ = e2d.HydroParams(nx=8, ny=8, precision=prec)
     got = e2d.pointwise(p_, e2d.FN_CONV_GT, np.stack([a, d], axis=1))[:, 0] == 1.0
     bad = np.nonzero(got != want)[0]

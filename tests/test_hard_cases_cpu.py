@@ -73,15 +73,17 @@ def test_division_by_root_cases_sit_next_to_midpoints(p, dt):
         assert abs(k - nearest_mid) < Fraction(1, 1 << (p - 6))
 
 
-def test_stop_test_cases_sit_next_to_the_tolerance_midpoint():
-    tol = float(np.float32(1e-6))
-    a, d, above = hc.quotients_next_to_midpoint(tol, 2000, seed=7)
+@pytest.mark.parametrize("p, dt, bound", [(53, np.float64, 70), (24, np.float32, 36)])
+def test_stop_test_cases_sit_next_to_the_tolerance_midpoint(p, dt, bound):
+    tol = dt(np.float32(1e-6))
+    a, d, above = hc.quotients_next_to_midpoint(float(tol), 2000, seed=7, max_r=1 << (20 if p == 53 else 6), p=p)
+    assert np.array_equal(a.astype(dt).astype(np.float64), a) and np.array_equal(d.astype(dt).astype(np.float64), d)
     h = np.spacing(tol) / 2
-    mu = Fraction(tol) + Fraction(h)
-    assert Fraction(np.nextafter(tol, 1.0)) - Fraction(tol) == 2 * Fraction(h)
+    mu = Fraction(float(tol)) + Fraction(float(h))
+    assert Fraction(float(np.nextafter(tol, dt(1.0)))) - Fraction(float(tol)) == 2 * Fraction(float(h))
     for av, dv, up in zip(a.tolist(), d.tolist(), above.tolist()):
         q = Fraction(av) / Fraction(dv)
-        # within 2^-70 of mu relative: 2^-17 of an ulp from the rounding boundary
-        assert (q > mu) == up and abs(q - mu) < mu * Fraction(1, 1 << 70)
-        assert ((av / dv) > tol) == up
+        # within 2^-bound of mu relative: a small fraction of an ulp from the rounding boundary
+        assert (q > mu) == up and abs(q - mu) < mu * Fraction(1, 1 << bound)
+        assert ((dt(av) / dt(dv)) > tol) == up
     assert 0.3 < above.mean() < 0.7

@@ -158,12 +158,18 @@ __device__ __forceinline__ void source_cell(const StepArgs<P>& a, int ci, bool c
 
 template <class P>
 __device__ __forceinline__ void prim_of(const Hydro<typename P::T>& h, const typename P::S raw[4],
-                                        typename P::T q[4]) {
+                                        typename P::T q[4], typename P::T& irho) {
   using T = typename P::T;
   T u[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) u[v] = static_cast<T>(raw[v]);
-  cons_to_prim<T, typename P::M>(h, u, q);
+  cons_to_prim<T, typename P::M>(h, u, q, irho);
+}
+template <class P>
+__device__ __forceinline__ void prim_of(const Hydro<typename P::T>& h, const typename P::S raw[4],
+                                        typename P::T q[4]) {
+  typename P::T irho;
+  prim_of<P>(h, raw, q, irho);
 }
 
 // kMinmod: instantiated for slope_type == 1 (the benchmark's minmod), whose
@@ -200,6 +206,7 @@ __device__ __forceinline__ void step_fused_body(const StepArgs<P>& a) {
   double invdt_cur = 0.0;
   if (a.dt_from_device) invdt_cur = invdt_part(a.scal, a.slot_cur);
   T qa[4], qb[4], qc[4];
+  T irb, irc;  // 1/rho of rows r and r+1 as the primitive conversion forms it, for the trace
   S raw[4], raw_next[4], raw_b[4];
   fetch_cell<P>(a, cl, j0 - 2, plain, raw);
   fetch_cell<P>(a, cl, j0 - 1, plain, raw_b);
@@ -207,7 +214,7 @@ __device__ __forceinline__ void step_fused_body(const StepArgs<P>& a) {
   source_cell<P>(a, cl, col_ok, j0 - 2, plain, raw);
   prim_of<P>(h, raw, qa);
   source_cell<P>(a, cl, col_ok, j0 - 1, plain, raw_b);
-  prim_of<P>(h, raw_b, qb);
+  prim_of<P>(h, raw_b, qb, irb);
 
   // dt of this step (serial_ref/m_run.f90:308, 384-385)
   U dt;
@@ -268,7 +275,7 @@ __device__ __forceinline__ void step_fused_body(const StepArgs<P>& a) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) rows_in[(slot_w * 4 + v) * 64 + lane] = raw[v];
 #endif
-    prim_of<P>(h, raw, qc);
+    prim_of<P>(h, raw, qc, irc);
 
     // slopes and trace sources of row r
     T qxm[4], qxp[4], dqx[4], dqy[4], s[4];
@@ -278,7 +285,7 @@ __device__ __forceinline__ void step_fused_body(const StepArgs<P>& a) {
       qxp[v] = from_right(qb[v]);
     }
     cell_slopes<kMinmod>(h, qb, qxp, qxm, qc, qa, dqx, dqy);
-    trace_sources<T, M>(h, qb, dqx, dqy, s);
+    trace_sources<T, M>(h, qb, dqx, dqy, s, irb);
 
     // y-face between rows r-1 and r
     if (r >= j0) {
@@ -337,6 +344,7 @@ __device__ __forceinline__ void step_fused_body(const StepArgs<P>& a) {
       fy_prev[v] = fy_cur[v];
       fx_prev[v] = fx_cur[v];
     }
+    irb = irc;
 #if E2D_FUSED_UOLD_LDS
     slot_w = slot_w == 2 ? 0 : slot_w + 1;
 #endif

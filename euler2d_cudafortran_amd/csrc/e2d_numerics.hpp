@@ -254,17 +254,25 @@ template <class T>
 E2D_HDI T signed_by(bool positive, T x) { return positive ? x : -x; }
 
 // Conservative -> primitive (computePrimitives + eos, m_utils.f90:35-64, 15-29).
+// irho: M::recip of the density, the reciprocal its three quotients share,
+// for later quotients by the same density (div_inv)
 template <class T, class M>
-E2D_HDI void cons_to_prim(const Hydro<T>& h, const T u[4], T q[4]) {
+E2D_HDI void cons_to_prim(const Hydro<T>& h, const T u[4], T q[4], T& irho) {
   const T rho = fmax2(u[0], h.smallr);
-  const T vx = M::div(u[2], rho);
-  const T vy = M::div(u[3], rho);
+  irho = M::recip(rho);
+  const T vx = M::div_inv(u[2], rho, irho);
+  const T vy = M::div_inv(u[3], rho, irho);
   const T kinetic = T(0.5) * (vx * vx + vy * vy);
-  const T eint = M::div(u[1], rho) - kinetic;
+  const T eint = M::div_inv(u[1], rho, irho) - kinetic;
   q[0] = rho;
   q[1] = fmax2((h.gamma0 - T(1)) * rho * eint, rho * h.smallp);
   q[2] = vx;
   q[3] = vy;
+}
+template <class T, class M>
+E2D_HDI void cons_to_prim(const Hydro<T>& h, const T u[4], T q[4]) {
+  T irho;
+  cons_to_prim<T, M>(h, u, q, irho);
 }
 
 // Sound speed of a primitive state (second half of eos) as cons_to_prim
@@ -335,15 +343,21 @@ E2D_HDI void cell_slopes(const Hydro<T>& h, const T q[4], const T qxp[4],
 
 // Hancock predictor source terms of a cell (trace_unsplit_2d_along_dir,
 // m_utils.f90:213-216): s = (sr0, sp0, su0, sv0).
+// irho: M::recip(q[0]), as cons_to_prim leaves it
 template <class T, class M>
 E2D_HDI void trace_sources(const Hydro<T>& h, const T q[4], const T dqx[4],
-                                              const T dqy[4], T s[4]) {
+                                              const T dqy[4], T s[4], T irho) {
   const T r = q[0], p = q[1], u = q[2], v = q[3];
   const T div_vel = dqx[2] + dqy[3];
   s[0] = -u * dqx[0] - v * dqy[0] - div_vel * r;
   s[1] = -u * dqx[1] - v * dqy[1] - div_vel * h.gamma0 * p;
-  s[2] = -u * dqx[2] - v * dqy[2] - M::div(dqx[1], r);
-  s[3] = -u * dqx[3] - v * dqy[3] - M::div(dqy[1], r);
+  s[2] = -u * dqx[2] - v * dqy[2] - M::div_inv(dqx[1], r, irho);
+  s[3] = -u * dqx[3] - v * dqy[3] - M::div_inv(dqy[1], r, irho);
+}
+template <class T, class M>
+E2D_HDI void trace_sources(const Hydro<T>& h, const T q[4], const T dqx[4],
+                                              const T dqy[4], T s[4]) {
+  trace_sources<T, M>(h, q, dqx, dqy, s, M::recip(q[0]));
 }
 
 // Face state of a cell: q + side*dq/2 + s*dt_dir/2 with only the density

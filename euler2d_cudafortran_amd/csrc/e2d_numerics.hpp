@@ -79,17 +79,21 @@ struct ExactMath {
   static E2D_HDI bool quot_gt(T a, T d, T tol, T) { return div(a, d) > tol; }
 };
 #if defined(__HIPCC__)
+// Its quotient is a times the native reciprocal of b: recip(b) is that
+// reciprocal (recip_near and sqrt_pos_inv form it too, whatever the start),
+// and div_inv(a, b, r) = a r, the same bits as div(a, b).
 struct FastMath {
   static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
   static __device__ __forceinline__ float sqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
   static __device__ __forceinline__ float sqrt_pos(float a) { return __builtin_amdgcn_sqrtf(a); }
   static __device__ __forceinline__ float sqrt_pos_inv(float a, float& inv) {
-    inv = 0.0f;
-    return sqrt_pos(a);
+    const float s = sqrt_pos(a);
+    inv = recip(s);
+    return s;
   }
-  static __device__ __forceinline__ float div_inv(float a, float b, float) { return div(a, b); }
-  static __device__ __forceinline__ float recip(float) { return 0.0f; }
-  static __device__ __forceinline__ float recip_near(float, float) { return 0.0f; }
+  static __device__ __forceinline__ float div_inv(float a, float, float r) { return a * r; }
+  static __device__ __forceinline__ float recip(float b) { return __builtin_amdgcn_rcpf(b); }
+  static __device__ __forceinline__ float recip_near(float b, float) { return recip(b); }
   static __device__ __forceinline__ bool quot_gt(float a, float d, float tol, float) { return div(a, d) > tol; }
 };
 

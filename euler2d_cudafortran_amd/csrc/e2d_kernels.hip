@@ -80,6 +80,15 @@ struct StepArgs {
   int nshort, short_rows, short_base, nitems;
   int chunk;  // items per chunk dealt to one XCD (see fused_segment)
   int nstrips, tile;  // items are numbered over tiles of `tile` adjacent strips (see tile_item)
+  // Packed last strip: when the last strip has at most 12 (28) output columns,
+  // its wavefronts are split into groups of pk_w = 16 (32) lanes, each group on
+  // its own segment of pk_rows rows, so the strip takes a quarter (half) of the
+  // wavefronts.  Items [nlong - npk, nlong) are its wavefronts; segment k of the
+  // strip covers rows [2 + k pk_rows, 2 + (k+1) pk_rows), the last one moved
+  // back to end at ny + 2 (it overlaps its neighbour, whose rows it then
+  // computes again with the same bits).  pk_w == 0: no packing; the last
+  // strip's index is nstrips (the regular strips are 0 .. nstrips - 1).
+  int pk_w, pk_rows, pk_nseg, npk;
 };
 
 // Item k of a tiled numbering of nstrips strips x nper segments each: tiles of
@@ -104,12 +113,28 @@ __device__ __forceinline__ void tile_item(int k, int nper, int nstrips, int tile
 // ends its share on short segments, which shortens the tail where the device
 // runs out of wavefronts.
 template <class P>
-__device__ __forceinline__ bool fused_segment(const StepArgs<P>& a, int b, int wave, int& strip, int& j0,
-                                              int& j1) {
+__device__ __forceinline__ bool fused_segment(const StepArgs<P>& a, int b, int wave, int lane, int& strip, int& j0,
+                                              int& j1, int& group_w) {
   int item;
   bool is_long;
   if (!dealt_item(b, wave, kWavesPerBlock, a.nlong, a.nitems, a.chunk, item, is_long)) return false;
   int seg;
+  group_w = 64;
+  if (is_long && item >= a.nlong - a.npk) {
+    // a wavefront of the packed last strip: group lane / pk_w takes segment
+    // (item - first) * (64 / pk_w) + group; a group past the last segment
+    // repeats the last one
+    group_w = a.pk_w;
+    strip = a.nstrips;
+    seg = min((item - (a.nlong - a.npk)) * (64 / a.pk_w) + lane / a.pk_w, a.pk_nseg - 1);
+    j0 = 2 + seg * a.pk_rows;
+    j1 = j0 + a.pk_rows;
+    if (j1 > a.g.ny + 2) {
+      j1 = a.g.ny + 2;
+      j0 = j1 - a.pk_rows;
+    }
+    return true;
+  }
   if (is_long) {
     tile_item(item, a.nsegs, a.nstrips, a.tile, strip, seg);
     j0 = 2 + seg * a.seg_rows;
@@ -184,19 +209,26 @@ __device__ __forceinline__ void step_fused_body(const StepArgs<P>& a) {
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const Grid& g = a.g;
-  int strip, j0, j1;
-  if (!fused_segment(a, blockIdx.x, wave, strip, j0, j1)) return;  // whole wavefront: no barrier here
-  const int ci = strip * kStripOut + lane;  // column of this lane
+  int strip, j0, j1, group_w;
+  if (!fused_segment(a, blockIdx.x, wave, lane, strip, j0, j1, group_w)) return;  // whole wavefront: no barrier here
+  // lanes per column group: 64, or 16 / 32 in the packed last strip, whose
+  // groups march different rows (j0, j1 per lane, j1 - j0 the same for all)
+  const int col = lane & (group_w - 1);
+  const int ci = strip * kStripOut + col;  // column of this lane
   const bool col_ok = ci < g.isize;
   const int cl = col_ok ? ci : g.isize - 1;  // clamped column for loads
-  const bool out_lane = (lane >= 2) && (lane < 62) && (ci <= g.nx + 1);
-  // columns strip*60 .. strip*60+63 and rows j0-2 .. j1+1 all interior
+  const bool out_lane = (col >= 2) && (col < group_w - 2) && (ci <= g.nx + 1);
+  // columns strip*60 .. strip*60+63 and rows j0-2 .. j1+1 all interior (never
+  // in the packed strip, which holds the last columns)
 #if E2D_FUSED_PLAIN
-  const bool plain = __builtin_amdgcn_readfirstlane(strip * kStripOut >= 2 && strip * kStripOut + 63 <= g.nx + 1 &&
-                                                    j0 - 2 >= 2 && j1 + 1 <= g.ny + 1 ? 1 : 0) != 0;
+  const bool plain = __builtin_amdgcn_readfirstlane(group_w == 64 && strip * kStripOut >= 2 &&
+                                                    strip * kStripOut + 63 <= g.nx + 1 && j0 - 2 >= 2 &&
+                                                    j1 + 1 <= g.ny + 1 ? 1 : 0) != 0;
 #else
   const bool plain = false;
 #endif
+  // iterations r = j0 - 1 .. j1 of the row loop, the same number on every lane
+  const int n_it = __builtin_amdgcn_readfirstlane(j1 - j0 + 2);
 
   const Hydro<T>& h = a.hT;
   U rate_max = U(0);
@@ -241,19 +273,20 @@ __device__ __forceinline__ void step_fused_body(const StepArgs<P>& a) {
   // (raw storage: S may have a constructor, which __shared__ does not allow)
   __shared__ __align__(16) unsigned char rows_in_mem[kWavesPerBlock * 3 * 4 * 64 * sizeof(S)];
   S* const rows_in = reinterpret_cast<S*>(rows_in_mem) + wave * (3 * 4 * 64);  // [slot][v][lane]
-  int slot_w = j0 % 3;  // slot of row r+1
+  int slot_w = 0;  // slot of row r+1; the next slot (mod 3) holds row r-1, written two iterations back
 #endif
   // Variable 0 of row r+2 of the input and of row r-1 of the output in the
   // lane's clamped column cl (ci on every lane that stores), moved one row on
   // per iteration: no row-index product per access.
   const S* in_row = a.uin + at(g, cl, j0 + 1, 0);
   S* out_row = a.uout + at(g, cl, j0 - 2, 0);
-  for (int r = j0 - 1; r <= j1; ++r, in_row += g.pitch, out_row += g.pitch) {
+  for (int it = 0; it < n_it; ++it, in_row += g.pitch, out_row += g.pitch) {
+    const int r = j0 - 1 + it;
     // software pipeline: row r+1 was fetched one iteration ago; fetch row r+2
     // now, so its load is in flight during the Riemann solves
 #pragma unroll
     for (int v = 0; v < 4; ++v) raw[v] = raw_next[v];
-    if (r + 2 <= j1 + 1) {
+    if (it < n_it - 1) {  // r + 2 <= j1 + 1
       if (plain) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) raw_next[v] = in_row[v * g.plane];
@@ -262,7 +295,7 @@ __device__ __forceinline__ void step_fused_body(const StepArgs<P>& a) {
       }
     }
     S uold[4];
-    const bool do_update = (r >= j0 + 1);
+    const bool do_update = it >= 2;  // r >= j0 + 1
 #if !E2D_FUSED_UOLD_LDS
     if (do_update && out_lane) {
 #pragma unroll
@@ -288,7 +321,7 @@ __device__ __forceinline__ void step_fused_body(const StepArgs<P>& a) {
     trace_sources<T, M>(h, qb, dqx, dqy, s, irb);
 
     // y-face between rows r-1 and r
-    if (r >= j0) {
+    if (it >= 1) {  // r >= j0
       T ymin[4], f[4];
       face_state(h, qb, dqy, s, dtdy, false, ymin);
       riemann_y<T, M, SOLVER>(h, ymax_prev, ymin, f);
@@ -298,7 +331,7 @@ __device__ __forceinline__ void step_fused_body(const StepArgs<P>& a) {
     face_state(h, qb, dqy, s, dtdy, true, ymax_prev);
 
     // x-face between columns ci-1 and ci, row r
-    if (r >= j0 && r < j1) {
+    if (it >= 1 && it < n_it - 1) {  // j0 <= r < j1
       T xmin[4], xmax[4], xl[4], gd[4], f[4];
       face_state(h, qb, dqx, s, dtdx, false, xmin);
       face_state(h, qb, dqx, s, dtdx, true, xmax);
@@ -317,7 +350,7 @@ __device__ __forceinline__ void step_fused_body(const StepArgs<P>& a) {
       if (out_lane) {
         const int jr = r - 1;
 #if E2D_FUSED_UOLD_LDS
-        const int slot_r = slot_w == 2 ? 0 : slot_w + 1;  // (r - 1) mod 3
+        const int slot_r = slot_w == 2 ? 0 : slot_w + 1;  // slot of row r - 1
 #pragma unroll
         for (int v = 0; v < 4; ++v) uold[v] = rows_in[(slot_r * 4 + v) * 64 + lane];
 #endif
@@ -358,8 +391,23 @@ __device__ __forceinline__ void step_fused_body(const StepArgs<P>& a) {
   }
 }
 
+// Resident wavefronts per SIMD the fused step is held at: 3 for FP64
+// arithmetic (168 VGPRs), 5 for FP32 (102), 4 for the native-arithmetic
+// variants (128), no cap for the tangent's dual numbers.  The packed last
+// strip's per-lane rows cost registers the compiler would otherwise take
+// (FP64 179, FP32 107: one wavefront fewer); held at these counts neither
+// spills.
+template <class P>
+constexpr int fused_waves() {
+  if constexpr (sizeof(typename P::T) > 8) return 1;
+  else if constexpr (sizeof(typename P::T) == 8) return 3;
+  else if constexpr (std::is_same<typename P::M, RangedMath>::value) return 5;
+  else return 4;
+}
+
 template <class P, int SOLVER, bool kMinmod = false>
-__global__ __launch_bounds__(64 * kWavesPerBlock) E2D_FUSED_ATTR void k_step_fused(const StepArgs<P> a) {
+__global__ __launch_bounds__(64 * kWavesPerBlock) __attribute__((amdgpu_waves_per_eu(fused_waves<P>())))
+E2D_FUSED_ATTR void k_step_fused(const StepArgs<P> a) {
   step_fused_body<P, SOLVER, kMinmod>(a);
 }
 
@@ -434,23 +482,42 @@ static int waves_per_simd(const void* kernel) {
 // values, E2D_FUSED_SHORT_ROWS the short length; the environment is read at
 // every launch.  Returns the number of blocks.
 template <class P>
-static int plan_segments(StepArgs<P>& a, int nstrips, int ny, int wps, int resident) {
+static int plan_segments(StepArgs<P>& a, int nstrips_all, int ny, int wps, int resident) {
+  // the packed last strip (see StepArgs): output columns left for it
+  const int last_cols = a.g.nx - kStripOut * (nstrips_all - 1);
+  int pk_w = 0;
+  if (nstrips_all >= 2 && env_int("E2D_FUSED_PACK", 1) != 0) pk_w = last_cols <= 12 ? 16 : (last_cols <= 28 ? 32 : 0);
+  const int nstrips = pk_w ? nstrips_all - 1 : nstrips_all;
+  const int groups = pk_w ? 64 / pk_w : 1;  // segments per packed wavefront
   int R = 0, L = 0;
   const bool forced = env_int("E2D_FUSED_SEG_ROWS", 0) >= 1 || env_int("E2D_FUSED_LONG_PCT", -1) >= 0;
-  const int round = resident / std::max(1, nstrips);  // long segments per strip in one round
-  if (!forced && round >= 1) {
+  if (!forced && resident >= nstrips) {
     // the long segments' share of the rows: 80% for the FP64 kernels (3
     // resident wavefronts per SIMD), 88% for the float-arithmetic ones (4-5):
     // measured at 2048^2, FP64 210-213 us at 80% against 214-218 at 76, 84 and
     // 88%; FP32 132.5 at 88% against 137 at 80%, fp32_native 104.5 against 107
     const int cover = env_int("E2D_FUSED_COVER_PCT", wps >= 4 ? 88 : 80);
     const double long_rows = std::min(100, std::max(10, cover)) / 100.0 * ny;
-    int k = 1;
-    while (long_rows / (k * round) > 40.0) ++k;
-    R = static_cast<int>(long_rows / (k * round) + 0.5);
-    L = k * round;
-    if (R < 8) R = 0;  // small grids: the percentage plan below
-    while (R > 0 && L * R > ny) --R;
+    // the packed wavefronts, each as long as a long segment, take their slots
+    // out of the rounds first (a second pass once their number is known)
+    int npk = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+      auto per_strip = [&](int k) { return k * ((resident - (npk + k - 1) / k) / nstrips); };
+      int k = 1;
+      while (per_strip(k) >= 1 && long_rows / per_strip(k) > 40.0) ++k;
+      L = per_strip(k);
+      if (L < 1) {
+        R = 0;
+        break;
+      }
+      R = static_cast<int>(long_rows / L + 0.5);
+      if (R < 8) R = 0;  // small grids: the percentage plan below
+      while (R > 0 && L * R > ny) --R;
+      if (R == 0 || !pk_w) break;
+      const int pk = ((ny + R - 1) / R + groups - 1) / groups;
+      if (pk == npk) break;
+      npk = pk;
+    }
   }
   if (R == 0) {
     R = fused_seg_rows(ny, wps);
@@ -461,7 +528,11 @@ static int plan_segments(StepArgs<P>& a, int nstrips, int ny, int wps, int resid
   const int covered = std::min(L * R, ny);
   a.seg_rows = R;
   a.nsegs = L;
-  a.nlong = nstrips * L;
+  a.pk_w = pk_w;
+  a.pk_rows = std::min(R, ny);
+  a.pk_nseg = pk_w ? (ny + a.pk_rows - 1) / a.pk_rows : 0;
+  a.npk = pk_w ? (a.pk_nseg + groups - 1) / groups : 0;
+  a.nlong = nstrips * L + a.npk;
   a.short_base = 2 + covered;
   a.short_rows = r;
   a.nshort = covered < ny ? (ny - covered + r - 1) / r : 0;

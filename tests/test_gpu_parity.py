@@ -158,8 +158,11 @@ def test_sizes_vs_oracle(gpu_lib, nx, ny, bc, impl):
 # long segments, strips per tile of the segment numbering (E2D_FUSED_SEG_ROWS /
 # _SHORT_ROWS / _LONG_PCT / _TILE); None = default.  At 130 columns there are
 # 3 strips: tile 2 leaves a last tile of one strip, 4 is clamped to 3.
-PLANS = [(None, None, None, None), (5, 2, 60, None), (3, 1, 0, None), (7, 3, 100, None), (1, 1, 75, None),
-         (40, 7, 90, None), (None, None, None, 2), (None, None, None, 3), (5, 2, 60, 4)]
+# (E2D_FUSED_SEG_ROWS, E2D_FUSED_SHORT_ROWS, E2D_FUSED_LONG_PCT, E2D_FUSED_TILE, E2D_FUSED_PACK);
+# at 130 columns the last strip has 10 output columns and is packed unless PACK is 0
+PLANS = [(None, None, None, None, None), (5, 2, 60, None, None), (3, 1, 0, None, None), (7, 3, 100, None, None),
+         (1, 1, 75, None, None), (40, 7, 90, None, None), (None, None, None, 2, None), (None, None, None, 3, None),
+         (5, 2, 60, 4, None), (None, None, None, None, 0), (5, 2, 60, None, 0)]
 
 
 @pytest.mark.parametrize("plan", PLANS)
@@ -168,7 +171,8 @@ def test_segment_plans_vs_oracle(gpu_lib, monkeypatch, plan, prec, kind):
     """Whatever the split of each strip's rows into long and short segments,
     and however they are spread over the XCDs, the fused step gives the
     oracle's bits (device-resident dt loop, 20 steps, 130 x 151)."""
-    for name, v in zip(("E2D_FUSED_SEG_ROWS", "E2D_FUSED_SHORT_ROWS", "E2D_FUSED_LONG_PCT", "E2D_FUSED_TILE"), plan):
+    for name, v in zip(("E2D_FUSED_SEG_ROWS", "E2D_FUSED_SHORT_ROWS", "E2D_FUSED_LONG_PCT", "E2D_FUSED_TILE",
+                        "E2D_FUSED_PACK"), plan):
         if v is None:
             monkeypatch.delenv(name, raising=False)
         else:

@@ -132,8 +132,10 @@ def test_stage_arrays_step0(gpu_lib, golden, impl):
 # ---------------------------------------------------------------------------
 # other sizes against the oracle (strip and segment edges, minimal grids)
 
+# 61 and 121 columns leave 1 output column to the last strip (packed in 16-lane
+# groups), 80 leaves 20 (32-lane groups), 60, 7 and 4 make a single strip
 SIZES = [(4, 4, (1, 1, 1, 1)), (61, 17, (1, 2, 3, 1)), (121, 33, (3, 3, 3, 3)),
-         (60, 16, (2, 2, 2, 2)), (7, 130, (1, 3, 2, 1))]
+         (60, 16, (2, 2, 2, 2)), (7, 130, (1, 3, 2, 1)), (80, 23, (3, 3, 2, 1)), (80, 41, (2, 1, 3, 3))]
 
 
 @pytest.mark.parametrize("impl", [0, 1, 2])

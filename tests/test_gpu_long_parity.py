@@ -103,3 +103,30 @@ def test_4096_bumps_fp64_vs_oracle(gpu_lib):
     next_dt = o.compute_dt(steps % 2)
     assert gdt == next_dt and ginvdt == o.invdt, (gdt, next_dt, ginvdt, o.invdt)
     assert _same(u, ref_u) and _same(u2, ref_u2), "4096^2 bumps run differs in some bit"
+
+
+@pytest.mark.parametrize("prec, kind", [(e2d.PREC_FP64, "f64"), (e2d.PREC_FP32, "f32")])
+def test_wide_grid_packed_last_strip_vs_oracle(gpu_lib, prec, kind):
+    """A large non-square grid whose last strip is packed in 32-lane groups
+    (4100 columns: 68 full strips and 20 columns left) and whose FP64 long
+    segments fill two rounds of resident wavefronts: 4100 x 2500, diagonal
+    problem, 20 steps of the device-resident dt loop, bit identity as above."""
+    nx, ny, steps = 4100, 2500, 20
+    p = e2d.HydroParams(nx=nx, ny=ny, precision=prec)
+    with e2d.HydroRun(p) as run:
+        run.init("diag")
+        run.make_boundaries(0)
+        run.run_steps(steps)
+        run.synchronize()
+        u, u2, gdt, ginvdt = run.download(0), run.download(1), run.get_dt(), run.invdt
+    o = Oracle(nx, ny, precision=kind)
+    o.init(IC_DIAG)
+    o.compute_dt(0)
+    o.make_boundaries(0)
+    for n in range(steps):
+        dt = o.compute_dt(n % 2)
+        o.godunov(n, dt)
+    ref_u, ref_u2 = o.get(U), o.get(U2)
+    next_dt = o.compute_dt(steps % 2)
+    assert gdt == next_dt and ginvdt == o.invdt, (gdt, next_dt, ginvdt, o.invdt)
+    assert _same(u, ref_u) and _same(u2, ref_u2), f"{nx} x {ny} {kind} run differs in some bit"

@@ -194,6 +194,28 @@ def test_segment_plans_vs_oracle(gpu_lib, monkeypatch, plan, prec, kind):
     assert _same(u2, o.get(U2)), _diff_report(u2, o.get(U2))
 
 
+@pytest.mark.parametrize("prec", [e2d.PREC_FP32_NATIVE, e2d.PREC_MIXED])
+@pytest.mark.parametrize("nx, ny", [(130, 151), (80, 64)])
+def test_native_variants_independent_of_the_plan(gpu_lib, monkeypatch, prec, nx, ny):
+    """fp32_native and mixed have no oracle of their own; their bits must not
+    depend on how the fused step splits the grid: the default plan (last strip
+    packed in 16- or 32-lane groups), the unpacked plan and a percentage plan
+    with short segments give the same state after 20 steps."""
+    out = []
+    for env in ({}, {"E2D_FUSED_PACK": "0"}, {"E2D_FUSED_SEG_ROWS": "3", "E2D_FUSED_LONG_PCT": "50"}):
+        for name in ("E2D_FUSED_PACK", "E2D_FUSED_SEG_ROWS", "E2D_FUSED_LONG_PCT"):
+            monkeypatch.delenv(name, raising=False)
+        for name, v in env.items():
+            monkeypatch.setenv(name, v)
+        with e2d.HydroRun(_params(nx, ny, 0, prec)) as run:
+            run.init("diag")
+            run.make_boundaries(0)
+            run.run_steps(20)
+            out.append((run.download(0), run.download(1), run.invdt))
+    for u, u2, invdt in out[1:]:
+        assert _same(u, out[0][0]) and _same(u2, out[0][1]) and invdt == out[0][2]
+
+
 @pytest.mark.parametrize("impl", [0, 1, 2])
 @pytest.mark.parametrize("prec, kind", [(e2d.PREC_FP64, "f64"), (e2d.PREC_FP32, "f32")])
 def test_hllc_runs_vs_oracle(gpu_lib, impl, prec, kind):
